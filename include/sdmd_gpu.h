@@ -1,0 +1,120 @@
+/*
+ * sdmd_gpu.h - C ABI of the MI355X (gfx950) sDMD event engine.
+ *
+ * The drop-in boundary for the reference's event-prediction hot path.  A
+ * context holds one or more systems resident in HBM (structure-of-arrays,
+ * FP64) and one HIP stream; every call returns SDMD_OK (0) or a negative
+ * SDMD_E* code (sdmd_host.h) and never exits the process.  Contexts are not
+ * thread-safe: one host thread per context.
+ *
+ * Entry points and the reference interfaces they replace:
+ *   sdmd_gpu_prepare      Prepare's bulk Predict(1..N) + CreateCBT,
+ *                         SGThread.c:47-62 (Predict 158-211, CBT.c:16-30)
+ *   sdmd_gpu_predict      Predict(int *renewList, ...), SGThread.c:158-211,
+ *                         declared DMD.h:666 (with ResetTarget,
+ *                         ToolFunctions.c:504-514)
+ *   sdmd_gpu_create_tree  CreateCBT(void), CBT.c:16-30, DMD.h:694
+ *   sdmd_gpu_update_tree  UpdateCBT(int *renewList), CBT.c:33-39, DMD.h:695
+ *   sdmd_gpu_pop          AssignJob(...), SGThread.c:237-259, DMD.h:667
+ *   sdmd_gpu_advance      UpdateData(double) + TimeForward(double),
+ *                         ToolFunctions.c:23-36, DMD.h:710-711
+ *   sdmd_gpu_do_event     DoEvent(struct AtomListStr*), Events.c:25-114,
+ *                         DMD.h:668 (InteractionEvent 798-1024, HBEvent
+ *                         275-393, ThermostatEvent 396-420,
+ *                         PBCandCrossCellEvent 248-272 -> PBC/LinkList)
+ *   sdmd_gpu_cell_move    LinkList("part", ...) after PBC("part", ...),
+ *                         Events.c:158-245, DMD.h:686-687
+ *   sdmd_gpu_run          the event loop SingleThreadRun, SGThread.c:67-156
+ *                         (AssignJob, UpdateData, TimeForward, HazardCheck
+ *                         215-235, DoEvent, Predict, UpdateCBT), run on the
+ *                         device for a given number of pops per system
+ * renew lists follow the reference's convention: renew[0] = count, atom ids
+ * from renew[1], zero-terminated.
+ */
+#ifndef SDMD_GPU_H
+#define SDMD_GPU_H
+
+#include <stdint.h>
+#include "sdmd_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sdmd_gpu_ctx sdmd_gpu_ctx;
+
+enum { SDMD_MODE_PLAIN = 0 };
+
+typedef struct {
+    int32_t mode;          /* SDMD_MODE_PLAIN: FP64, bit-exact with the reference */
+    int32_t block;         /* threads per system workgroup (64..1024, multiple of 64); 0 = 256 */
+    int32_t trace;         /* record every pop (parity checks) */
+    int32_t reserved;
+    int64_t trace_capacity;
+    int64_t hash_every;    /* state hash every n pops (0 = off) */
+} sdmd_gpu_opts;
+
+/* one popped event (same layout as the reference trace dump) */
+typedef struct {
+    int32_t target, partner, type, sub, exec, nrenew;
+    double dt, pot;
+    int64_t rand_calls;
+    struct {
+        int32_t atom, partner, type, sub;
+        int64_t counter, pcounter;
+        double time, pot;
+    } r[2];
+} sdmd_gpu_pop_rec;
+
+typedef struct {
+    int64_t pops, frame, recal;        /* pops, executed events, stale re-predictions */
+    int64_t per_type[16];              /* executed events per EEventType */
+    int64_t pair_evals, candidates;    /* pair-time evaluations, candidates scanned */
+    int64_t normals_used;              /* thermostat stream position */
+    double currenttime;
+    int32_t error;                     /* device error code (0 = none) */
+    int32_t pad;
+} sdmd_gpu_stats;
+
+typedef struct {
+    int32_t target, partner, type, sub;
+    double time, potential;
+} sdmd_gpu_event;
+
+/* Upload systems (a private device copy of every array of each desc). */
+int sdmd_gpu_open(const sdmd_system_desc *const *systems, int32_t nsys, const sdmd_gpu_opts *opts,
+                  sdmd_gpu_ctx **out);
+int sdmd_gpu_close(sdmd_gpu_ctx *ctx);
+
+/* Thermostat draw stream of one system: standard-normal values produced by
+ * sdmd_host_normals, consumed in the reference's order (one per predicted
+ * atom, three per thermostat event).  Replaces the unread remainder. */
+int sdmd_gpu_set_normals(sdmd_gpu_ctx *ctx, int32_t sys, const double *z, int64_t n);
+
+int sdmd_gpu_prepare(sdmd_gpu_ctx *ctx);
+int sdmd_gpu_run(sdmd_gpu_ctx *ctx, int64_t pops_per_system);
+
+int sdmd_gpu_predict(sdmd_gpu_ctx *ctx, int32_t sys, const int32_t *renew);
+int sdmd_gpu_create_tree(sdmd_gpu_ctx *ctx, int32_t sys);
+int sdmd_gpu_update_tree(sdmd_gpu_ctx *ctx, int32_t sys, const int32_t *renew);
+int sdmd_gpu_pop(sdmd_gpu_ctx *ctx, int32_t sys, sdmd_gpu_event *out);
+int sdmd_gpu_advance(sdmd_gpu_ctx *ctx, int32_t sys, double dt);
+int sdmd_gpu_do_event(sdmd_gpu_ctx *ctx, int32_t sys, int32_t target, int32_t partner);
+int sdmd_gpu_cell_move(sdmd_gpu_ctx *ctx, int32_t sys, int32_t atom);
+
+/* Copy the device state of one system back into the caller's arrays (same
+ * sizes as the desc it was opened with). */
+int sdmd_gpu_download(sdmd_gpu_ctx *ctx, int32_t sys, sdmd_system_desc *out);
+int sdmd_gpu_get_tree(sdmd_gpu_ctx *ctx, int32_t sys, int32_t *node, int32_t *leaf);
+int sdmd_gpu_stats_get(sdmd_gpu_ctx *ctx, int32_t sys, sdmd_gpu_stats *out);
+/* Trace of the last runs (when opts.trace): copies up to max records. */
+int sdmd_gpu_get_trace(sdmd_gpu_ctx *ctx, int32_t sys, sdmd_gpu_pop_rec *out, int64_t max, int64_t *n);
+int sdmd_gpu_get_hashes(sdmd_gpu_ctx *ctx, int32_t sys, int64_t *out, int64_t max, int64_t *n);
+/* Time of the last sdmd_gpu_run / sdmd_gpu_prepare launch on the context's
+ * stream, measured with HIP events (ms). */
+int sdmd_gpu_last_kernel_ms(sdmd_gpu_ctx *ctx, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -55,8 +55,9 @@ const sdmd_system_desc *sdmd_host_desc(sdmd_host_system *s);
 
 /* Next n standard-normal draws of the reference's Maxwell-Boltzmann sampler
  * (Marsaglia polar + coin flip on glibc-compatible rand()), continuing this
- * system's RNG stream. */
-int sdmd_host_normals(sdmd_host_system *s, double *out, long n);
+ * system's RNG stream.  calls_out (may be NULL) receives the number of rand()
+ * calls made since srand() after each draw. */
+int sdmd_host_normals(sdmd_host_system *s, double *out, long n, int64_t *calls_out);
 
 /* glibc TYPE_3 rand() restated: seed, draw, and position. */
 typedef struct { int32_t r[34]; int f, b; int64_t calls; } sdmd_rand_state;

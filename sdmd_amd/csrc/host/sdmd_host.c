@@ -953,7 +953,9 @@ static int sd_energy_min(sdmd_host_system *s, long step_num) {
         if (c[3 * i + 1] > maxy) maxy = c[3 * i + 1];
         if (c[3 * i + 2] > maxz) maxz = c[3 * i + 2];
     }
-    const double G = 12.5;
+    double G = 12.5;
+    if (N < 1) return SDMD_OK;
+    while ((maxx - minx) / G * ((maxy - miny) / G) * ((maxz - minz) / G) > 64.0 * N + 1e6) G *= 2;
     int gx = (int)((maxx - minx) / G) + 1, gy = (int)((maxy - miny) / G) + 1, gz = (int)((maxz - minz) / G) + 1;
     long ng = (long)gx * gy * gz;
     int *head = malloc(sizeof(int) * ng), *next = malloc(sizeof(int) * (N + 1));
@@ -1104,9 +1106,12 @@ int sdmd_host_prepare(sdmd_host_system *s, long max_steps) {
     return link_list_full(s);
 }
 
-int sdmd_host_normals(sdmd_host_system *s, double *out, long n) {
+int sdmd_host_normals(sdmd_host_system *s, double *out, long n, int64_t *calls_out) {
     if (!s || (!out && n > 0) || n < 0) return SDMD_EINVAL;
-    for (long k = 0; k < n; k++) out[k] = mb_normal(&s->rng);
+    for (long k = 0; k < n; k++) {
+        out[k] = mb_normal(&s->rng);
+        if (calls_out) calls_out[k] = s->rng.calls;
+    }
     return SDMD_OK;
 }
 

@@ -1,0 +1,55 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+GOLDEN = os.path.join(HERE, "golden")
+for p in (ROOT, HERE, GOLDEN):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+CASES = ["ala10ext", "ala10helix", "mix4x20", "ala10nve"]
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+    config.addinivalue_line("markers", "slow: longer CPU test")
+
+
+def golden(name):
+    out = {}
+    for part in ("setup", "topo", "trace", "input"):
+        with np.load(os.path.join(GOLDEN, "%s_%s.npz" % (name, part)), allow_pickle=False) as z:
+            out[part] = {k: z[k] for k in z.files}
+    return out
+
+
+_SYSTEMS = {}
+
+
+def golden_system(name):
+    """Host-prepared System for a golden case (from the stored input text)."""
+    from sdmd_amd.engine import System
+
+    g = golden(name)
+    seed = int(g["setup"]["seed"])
+    return System.from_text(str(g["input"]["gro"]), str(g["input"]["parameter"]), seed=seed), g
+
+
+def gpu_available():
+    try:
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
+        n = ctypes.c_int(0)
+        return hip.hipGetDeviceCount(ctypes.byref(n)) == 0 and n.value > 0
+    except OSError:
+        return False
+
+
+@pytest.fixture(scope="session")
+def require_gpu():
+    if not gpu_available():
+        pytest.fail("no HIP device visible: GPU tests must run on an MI355X")

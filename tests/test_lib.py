@@ -1,0 +1,49 @@
+"""The C-ABI library loads and exports every function include/*.h declares
+(no device calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+
+def declared_functions():
+    names = set()
+    for h in ("sdmd_host.h", "sdmd_gpu.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(sdmd_\w+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    from sdmd_amd import _lib
+
+    L = _lib.lib()
+    names = declared_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(_lib.HOST_SYMBOLS + _lib.GPU_SYMBOLS) == set(names)
+
+
+def test_gpu_open_rejects_bad_arguments_without_device():
+    from sdmd_amd import _lib
+
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    assert L.sdmd_gpu_open(None, 0, None, ctypes.byref(h)) == -1
+    assert L.sdmd_gpu_close(None) == -1
+    assert L.sdmd_gpu_run(None, 10) == -1
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    from sdmd_amd import _lib
+
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
+    with pytest.raises(ImportError):
+        _lib.lib()
