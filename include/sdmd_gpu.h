@@ -15,11 +15,14 @@
  *                         ToolFunctions.c:504-514)
  *   sdmd_gpu_create_tree  CreateCBT(void), CBT.c:16-30, DMD.h:694
  *   sdmd_gpu_update_tree  UpdateCBT(int *renewList), CBT.c:33-39, DMD.h:695
- *   sdmd_gpu_pop          AssignJob(...), SGThread.c:237-259, DMD.h:667
+ *   sdmd_gpu_pop          AssignJob(...), SGThread.c:237-259, DMD.h:667, and
+ *                         HazardCheck(...), SGThread.c:215-235, DMD.h:671
+ *                         (the stale flag; counters do not change in between)
  *   sdmd_gpu_advance      UpdateData(double) + TimeForward(double),
  *                         ToolFunctions.c:23-36, DMD.h:710-711
  *   sdmd_gpu_do_event     DoEvent(struct AtomListStr*), Events.c:25-114,
- *                         DMD.h:668 (InteractionEvent 798-1024, HBEvent
+ *                         DMD.h:668, with the caller's frame and event-counter
+ *                         increments of SGThread.c:100-104 (InteractionEvent 798-1024, HBEvent
  *                         275-393, ThermostatEvent 396-420,
  *                         PBCandCrossCellEvent 248-272 -> PBC/LinkList)
  *   sdmd_gpu_cell_move    LinkList("part", ...) after PBC("part", ...),
@@ -79,6 +82,8 @@ typedef struct {
 typedef struct {
     int32_t target, partner, type, sub;
     double time, potential;
+    int32_t stale;                     /* HazardCheck: partner changed since prediction */
+    int32_t pad;
 } sdmd_gpu_event;
 
 /* Upload systems (a private device copy of every array of each desc). */

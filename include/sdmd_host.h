@@ -59,6 +59,10 @@ const sdmd_system_desc *sdmd_host_desc(sdmd_host_system *s);
  * calls made since srand() after each draw. */
 int sdmd_host_normals(sdmd_host_system *s, double *out, long n, int64_t *calls_out);
 
+/* The same draws, from a given stream position: srand(seed) followed by
+ * `skip` rand() calls (a system desc carries both as seed / rand_calls). */
+int sdmd_normal_stream(uint32_t seed, int64_t skip, double *out, long n, int64_t *calls_out);
+
 /* glibc TYPE_3 rand() restated: seed, draw, and position. */
 typedef struct { int32_t r[34]; int f, b; int64_t calls; } sdmd_rand_state;
 void sdmd_rand_seed(sdmd_rand_state *st, uint32_t seed);

@@ -17,7 +17,9 @@
  *   SaveData (DataSave.c)           -> disabled (no output files)
  * Nothing here changes the reference's arithmetic or control flow.
  */
+#define _POSIX_C_SOURCE 199309L
 #include <stdint.h>
+#include <time.h>
 #include "DMD.h"
 
 int __real_rand(void);
@@ -34,6 +36,8 @@ static long long g_hash_every = 1000;
 static int g_exec = 0;
 static FILE *g_trace = NULL, *g_hash = NULL;
 static char g_dir[1024] = ".";
+static struct timespec g_t0;   /* start of the event loop (after CreateCBT) */
+static int g_light = 0;        /* SDMD_LIGHT: no dumps, loop timing only */
 
 struct PopRec {
     int32_t target, partner, type, sub, exec, nrenew;
@@ -177,16 +181,22 @@ void __wrap_CreateCBT(void) {
     if (e) snprintf(g_dir, sizeof g_dir, "%s", e);
     if ((e = getenv("SDMD_MAX_POPS"))) g_max_pops = atoll(e);
     if ((e = getenv("SDMD_HASH_EVERY"))) g_hash_every = atoll(e);
-    write_state("setup.bin");
-    write_topology();
+    if ((e = getenv("SDMD_LIGHT"))) g_light = atoi(e);
+    if (!g_light) {
+        write_state("setup.bin");
+        write_topology();
+    }
     __real_CreateCBT();
-    FILE *f = open_out("tree.bin", "wb");
-    fwrite(CBT.node, sizeof(int), 2 * atomnum, f);
-    fwrite(CBT.leaf, sizeof(int), atomnum + 1, f);
-    fclose(f);
-    g_trace = open_out("trace.bin", "wb");
-    g_hash = open_out("hash.bin", "wb");
-    if (g_max_pops == 0) { fclose(g_trace); fclose(g_hash); exit(0); }
+    if (!g_light) {
+        FILE *f = open_out("tree.bin", "wb");
+        fwrite(CBT.node, sizeof(int), 2 * atomnum, f);
+        fwrite(CBT.leaf, sizeof(int), atomnum + 1, f);
+        fclose(f);
+        g_trace = open_out("trace.bin", "wb");
+        g_hash = open_out("hash.bin", "wb");
+    }
+    if (g_max_pops == 0) { if (g_trace) { fclose(g_trace); fclose(g_hash); } exit(0); }
+    clock_gettime(CLOCK_MONOTONIC, &g_t0);
 }
 
 void __wrap_UpdateData(double dt) {
@@ -224,16 +234,24 @@ void __wrap_UpdateCBT(int *renew) {
         g_cur.r[n].pot = d->event.potential;
     }
     g_cur.nrenew = n;
-    fwrite(&g_cur, sizeof g_cur, 1, g_trace);
+    if (g_trace) fwrite(&g_cur, sizeof g_cur, 1, g_trace);
     g_pops++;
-    if (g_hash_every > 0 && g_pops % g_hash_every == 0) {
+    if (g_hash && g_hash_every > 0 && g_pops % g_hash_every == 0) {
         int64_t rec[2] = {g_pops, (int64_t)state_hash()};
         fwrite(rec, sizeof rec, 1, g_hash);
     }
     if (g_max_pops > 0 && g_pops >= g_max_pops) {
-        fclose(g_trace);
-        fclose(g_hash);
-        write_state("final.bin");
+        struct timespec t1;
+        clock_gettime(CLOCK_MONOTONIC, &t1);
+        double el = (double)(t1.tv_sec - g_t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - g_t0.tv_nsec);
+        FILE *tf = open_out("timing.txt", "w");
+        fprintf(tf, "pops %lld events %ld recal %ld seconds %.6f\n", g_pops, frame, countReCal, el);
+        fclose(tf);
+        if (g_trace) {
+            fclose(g_trace);
+            fclose(g_hash);
+            write_state("final.bin");
+        }
         exit(0);
     }
 }

@@ -39,11 +39,13 @@ class GpuStats(C.Structure):
 
 class GpuEvent(C.Structure):
     _fields_ = [("target", C.c_int32), ("partner", C.c_int32), ("type", C.c_int32),
-                ("sub", C.c_int32), ("time", C.c_double), ("potential", C.c_double)]
+                ("sub", C.c_int32), ("time", C.c_double), ("potential", C.c_double),
+                ("stale", C.c_int32), ("pad", C.c_int32)]
 
 
 # every symbol declared in include/sdmd_host.h and include/sdmd_gpu.h
 HOST_SYMBOLS = ["sdmd_host_load", "sdmd_host_prepare", "sdmd_host_desc", "sdmd_host_normals",
+                "sdmd_normal_stream",
                 "sdmd_rand_seed", "sdmd_rand_next", "sdmd_host_free", "sdmd_strerror"]
 GPU_SYMBOLS = ["sdmd_gpu_open", "sdmd_gpu_close", "sdmd_gpu_set_normals", "sdmd_gpu_prepare",
                "sdmd_gpu_run", "sdmd_gpu_predict", "sdmd_gpu_create_tree", "sdmd_gpu_update_tree",
@@ -66,6 +68,7 @@ def lib():
     L.sdmd_host_desc.restype = C.POINTER(SystemDesc)
     L.sdmd_host_desc.argtypes = [P]
     L.sdmd_host_normals.argtypes = [P, C.c_void_p, C.c_long, C.c_void_p]
+    L.sdmd_normal_stream.argtypes = [C.c_uint32, C.c_int64, C.c_void_p, C.c_long, C.c_void_p]
     L.sdmd_rand_seed.argtypes = [C.c_void_p, C.c_uint32]
     L.sdmd_rand_next.restype = C.c_int32
     L.sdmd_rand_next.argtypes = [C.c_void_p]

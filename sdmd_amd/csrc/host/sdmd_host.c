@@ -1115,6 +1115,18 @@ int sdmd_host_normals(sdmd_host_system *s, double *out, long n, int64_t *calls_o
     return SDMD_OK;
 }
 
+int sdmd_normal_stream(uint32_t seed, int64_t skip, double *out, long n, int64_t *calls_out) {
+    if ((!out && n > 0) || n < 0 || skip < 0) return SDMD_EINVAL;
+    sdmd_rand_state st;
+    sdmd_rand_seed(&st, seed);
+    for (int64_t k = 0; k < skip; k++) sdmd_rand_next(&st);
+    for (long k = 0; k < n; k++) {
+        out[k] = mb_normal(&st);
+        if (calls_out) calls_out[k] = st.calls;
+    }
+    return SDMD_OK;
+}
+
 /* ------------------------------------------------------------------ */
 /* flattening                                                         */
 /* ------------------------------------------------------------------ */

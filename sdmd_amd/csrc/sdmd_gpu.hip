@@ -1252,8 +1252,11 @@ __global__ __launch_bounds__(B) void k_op(Sys *systems, int sys, int op, const i
         if (op == OP_CREATE_TREE) cbt_create(S);
         else if (op == OP_UPDATE_TREE) { for (int k = 1; renew[k]; k++) cbt_update_run(S, renew[k]); }
         else if (op == OP_DO_EVENT) {
+            // SingleThreadRun, SGThread.c:100-104
             S.frame++;
             do_event(S, sm, a, b, &err);
+            S.evc[a]++;
+            if (b > 0) S.evc[b]++;
         } else if (op == OP_CELL_MOVE) pcc_event(S, a, &err);
         if (err) sm.err = err;
     }
@@ -1534,6 +1537,15 @@ extern "C" int sdmd_gpu_pop(sdmd_gpu_ctx *c, int32_t s, sdmd_gpu_event *out) {
         hipMemcpy(&out->time, S.evt + root, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(&out->potential, S.evpot + root, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
         return SDMD_EDEVICE;
+    out->stale = 0;
+    out->pad = 0;
+    if (out->partner > 0) {   // HazardCheck, SGThread.c:224-227
+        long long pc = 0, cnt = 0;
+        if (hipMemcpy(&pc, S.evpc + root, sizeof pc, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(&cnt, S.evc + out->partner, sizeof cnt, hipMemcpyDeviceToHost) != hipSuccess)
+            return SDMD_EDEVICE;
+        out->stale = pc < cnt;
+    }
     return SDMD_OK;
 }
 

@@ -73,11 +73,14 @@ class System:
         return self.arrays.desc()
 
     def normals(self, n):
-        """Next n thermostat draws of this system's RNG stream and the rand()
-        call count after each."""
+        """The first n thermostat draws of the event loop (the reference's
+        stream from the RNG position reached by setup) and the rand() call
+        count after each.  Stateless: every call returns the same stream."""
         z = np.zeros(n, np.float64)
         calls = np.zeros(n, np.int64)
-        check(_lib.lib().sdmd_host_normals(self._h, z.ctypes.data, n, calls.ctypes.data), "sdmd_host_normals")
+        sc = self.arrays.scalars
+        check(_lib.lib().sdmd_normal_stream(sc["seed"], sc["rand_calls"], z.ctypes.data, n, calls.ctypes.data),
+              "sdmd_normal_stream")
         return z, calls
 
     def close(self):
@@ -151,7 +154,7 @@ class GpuEngine:
         ev = GpuEvent()
         check(_lib.lib().sdmd_gpu_pop(self._h, sys, C.byref(ev)), "sdmd_gpu_pop")
         return {"target": ev.target, "partner": ev.partner, "type": ev.type, "sub": ev.sub,
-                "time": ev.time, "potential": ev.potential}
+                "time": ev.time, "potential": ev.potential, "stale": bool(ev.stale)}
 
     def advance(self, sys, dt):
         check(_lib.lib().sdmd_gpu_advance(self._h, sys, dt), "sdmd_gpu_advance")
