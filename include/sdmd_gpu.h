@@ -52,7 +52,7 @@ typedef struct {
     int32_t mode;          /* SDMD_MODE_PLAIN: FP64, bit-exact with the reference */
     int32_t block;         /* threads per system workgroup (64..1024, multiple of 64); 0 = 256 */
     int32_t trace;         /* record every pop (parity checks) */
-    int32_t reserved;
+    int32_t profile;       /* bit 0: accumulate per-phase device time */
     int64_t trace_capacity;
     int64_t hash_every;    /* state hash every n pops (0 = off) */
 } sdmd_gpu_opts;
@@ -115,6 +115,10 @@ int sdmd_gpu_stats_get(sdmd_gpu_ctx *ctx, int32_t sys, sdmd_gpu_stats *out);
 /* Trace of the last runs (when opts.trace): copies up to max records. */
 int sdmd_gpu_get_trace(sdmd_gpu_ctx *ctx, int32_t sys, sdmd_gpu_pop_rec *out, int64_t max, int64_t *n);
 int sdmd_gpu_get_hashes(sdmd_gpu_ctx *ctx, int32_t sys, int64_t *out, int64_t max, int64_t *n);
+/* Per-phase device time in seconds (pop, sweep, event, stage, gather,
+ * bond selection, evaluation, selection, HB scan, tail, tree) for a context
+ * opened with opts.profile = 1. */
+int sdmd_gpu_phase_times(sdmd_gpu_ctx *ctx, int32_t sys, double *out, int32_t n);
 /* Time of the last sdmd_gpu_run / sdmd_gpu_prepare launch on the context's
  * stream, measured with HIP events (ms). */
 int sdmd_gpu_last_kernel_ms(sdmd_gpu_ctx *ctx, double *ms);

@@ -27,7 +27,7 @@ class PopRec(C.Structure):
 
 class GpuOpts(C.Structure):
     _fields_ = [("mode", C.c_int32), ("block", C.c_int32), ("trace", C.c_int32),
-                ("reserved", C.c_int32), ("trace_capacity", C.c_int64), ("hash_every", C.c_int64)]
+                ("profile", C.c_int32), ("trace_capacity", C.c_int64), ("hash_every", C.c_int64)]
 
 
 class GpuStats(C.Structure):
@@ -51,7 +51,7 @@ GPU_SYMBOLS = ["sdmd_gpu_open", "sdmd_gpu_close", "sdmd_gpu_set_normals", "sdmd_
                "sdmd_gpu_run", "sdmd_gpu_predict", "sdmd_gpu_create_tree", "sdmd_gpu_update_tree",
                "sdmd_gpu_pop", "sdmd_gpu_advance", "sdmd_gpu_do_event", "sdmd_gpu_cell_move",
                "sdmd_gpu_download", "sdmd_gpu_get_tree", "sdmd_gpu_stats_get", "sdmd_gpu_get_trace",
-               "sdmd_gpu_get_hashes", "sdmd_gpu_last_kernel_ms"]
+               "sdmd_gpu_get_hashes", "sdmd_gpu_phase_times", "sdmd_gpu_last_kernel_ms"]
 
 
 def lib():
@@ -92,6 +92,7 @@ def lib():
     L.sdmd_gpu_stats_get.argtypes = [P, I, C.POINTER(GpuStats)]
     L.sdmd_gpu_get_trace.argtypes = [P, I, C.c_void_p, I64, C.POINTER(I64)]
     L.sdmd_gpu_get_hashes.argtypes = [P, I, C.c_void_p, I64, C.POINTER(I64)]
+    L.sdmd_gpu_phase_times.argtypes = [P, I, C.c_void_p, I]
     L.sdmd_gpu_last_kernel_ms.argtypes = [P, C.POINTER(D)]
     _lib = L
     return L

@@ -87,7 +87,16 @@ struct Sys {
     long long trace_cap, trace_n;
     long long *hashes;
     long long hash_every, hash_cap, hash_n;
+    unsigned long long *prof;   // per-phase wall-clock ticks, or null
+    int steps_sorted;           // every step list strictly increasing and non-zero
 };
+
+// phases timed when a context is opened with profiling on
+enum { PH_POP, PH_SWEEP, PH_EVENT, PH_STAGE, PH_GATHER, PH_BONDSEL, PH_EVAL, PH_SELECT, PH_HB, PH_TAIL,
+       PH_TREE, PH_NPH };
+
+#define PROF_MARK(S, ph, t) do { if ((S).prof && threadIdx.x == 0) { unsigned long long _n = wall_clock64(); \
+    (S).prof[ph] += _n - (t); (t) = _n; } } while (0)
 
 // ---------------------------------------------------------------- models
 __device__ __forceinline__ int type_change(int t, int direct) {   // AtomTypeChange, ToolFunctions.c:233-270
@@ -195,6 +204,30 @@ __device__ int find_pair(const Sys &S, int a1, int a2, int kind, double directio
     }
     f.lo = dmin;
     f.lopot = 0;
+    if (S.steps_sorted && nstep <= 16) {
+        // Every step list is strictly increasing and non-zero (checked at
+        // upload), so the walk below stops at the first k with
+        // d2 < d_k + tol, i.e. at the number of radii d2 has passed.  The
+        // radii are read with independent loads.
+        int ks = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            if (k < nstep) ks += (d2 >= S.sd[step0 + k] + tol) ? 1 : 0;
+        if (ks > 0) {
+            f.lo = S.sd[step0 + ks - 1];
+            f.lopot = S.se[step0 + ks - 1];
+        }
+        if (ks < nstep) {
+            const int k = ks;
+            f.hi = S.sd[step0 + k];
+            f.hipot = -1 * S.se[step0 + k];
+            f.acc = S.sacc[step0 + k];
+            if (k == 0) return 0;
+            if (kind == K_HB && k == nstep - 1 && S.hbbond[a1] > 0) return 1;
+            if (kind == K_NEIGHBOR && k == nstep - 1 && S.hbbond[a2] > 0) { f.hipot = 0; return 1; }
+            return cs;
+        }
+    } else
     for (int k = 0; k < nstep; k++) {
         const double sd = S.sd[step0 + k];
         if (sd == 0) break;
@@ -327,6 +360,8 @@ struct Smem {
     int ncand;
     int row_j[ROWCAP];
     int row_b[ROWCAP];
+    int dyn_j[SDMD_DYNSLOTS];
+    int dyn_b[SDMD_DYNSLOTS];
     // bond entries
     int bj[MAXBOND];
     double bt[MAXBOND], bpot[MAXBOND];
@@ -351,9 +386,8 @@ __device__ __forceinline__ int row_bits(const Smem &sm, const Sys &S, int j) {
             if (v == j) { bits = sm.row_b[m]; break; }
             if (v < j) lo = m + 1; else hi = m - 1;
         }
-        const int *dj = S.dynj + (size_t)A.i * SDMD_DYNSLOTS;
         for (int k = 0; k < SDMD_DYNSLOTS; k++)
-            if (dj[k] == j) { bits |= S.dynb[(size_t)A.i * SDMD_DYNSLOTS + k]; break; }
+            if (sm.dyn_j[k] == j) { bits |= sm.dyn_b[k]; break; }
         return bits;
     }
     return cmap_get(S, A.i, j);
@@ -685,6 +719,7 @@ template <int B>
 __device__ void predict_atom(Sys &S, Smem &sm, int i) {
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
+    unsigned long long pt = S.prof ? wall_clock64() : 0;
     if (tid == 0) {
         stage_atom(S, sm, i);
         sm.cnt_cand = 0;
@@ -697,9 +732,15 @@ __device__ void predict_atom(Sys &S, Smem &sm, int i) {
             sm.row_j[k] = S.cmap_j[S.cmap_off[i] + k];
             sm.row_b[k] = S.cmap_bits[S.cmap_off[i] + k];
         }
+    if (tid >= B - SDMD_DYNSLOTS) {
+        const int k = tid - (B - SDMD_DYNSLOTS);
+        sm.dyn_j[k] = S.dynj[(size_t)i * SDMD_DYNSLOTS + k];
+        sm.dyn_b[k] = S.dynb[(size_t)i * SDMD_DYNSLOTS + k];
+    }
     int err = 0;
     const int nb = sm.nb;
     if (nb > MAXBOND && tid == 0) sm.err = ERR_CELLCAP;
+    PROF_MARK(S, PH_STAGE, pt);
     // bonds (wave 0) and the 27-cell gather (wave 1, or wave 0 lanes 32.. when B == 64)
     if (tid < min(nb, MAXBOND)) eval_bond(S, sm, tid, &err);
     const int gwave = (B >= 128) ? 1 : 0;
@@ -716,6 +757,7 @@ __device__ void predict_atom(Sys &S, Smem &sm, int i) {
         sm.cellcnt[c] = cnt;
     }
     __syncthreads();
+    PROF_MARK(S, PH_GATHER, pt);
     if (wave == 0) {
         // bond selection
         double m = sm.A.t;
@@ -745,6 +787,7 @@ __device__ void predict_atom(Sys &S, Smem &sm, int i) {
         if (q < sm.cellcnt[c]) sm.cand_j[sm.celloff[c] + q] = sm.cellbuf[k];
     }
     __syncthreads();
+    PROF_MARK(S, PH_BONDSEL, pt);
     double calib = S.cutoffr * SDMD_EXTEND_RATIO;
     calib *= calib;
     int my_cand = 0, my_eval = 0;
@@ -757,6 +800,7 @@ __device__ void predict_atom(Sys &S, Smem &sm, int i) {
     if (my_cand) atomicAdd(&sm.cnt_cand, my_cand);
     if (my_eval) atomicAdd(&sm.cnt_eval, my_eval);
     __syncthreads();
+    PROF_MARK(S, PH_EVAL, pt);
     if (wave == 0) {
         double m = sm.A.t;
         int win = -1, inh = 0;
@@ -786,6 +830,7 @@ __device__ void predict_atom(Sys &S, Smem &sm, int i) {
         }
     }
     __syncthreads();
+    PROF_MARK(S, PH_SELECT, pt);
     if (sm.hbscan) {
         int my_hb = 0;
         for (int k = tid; k < ncand; k += B) {
@@ -804,6 +849,7 @@ __device__ void predict_atom(Sys &S, Smem &sm, int i) {
             }
         }
     }
+    PROF_MARK(S, PH_HB, pt);
     if (tid == 0) {
         hb_neighbor_time(S, sm.A, &err);
         unstage_atom(S, sm.A);
@@ -813,6 +859,7 @@ __device__ void predict_atom(Sys &S, Smem &sm, int i) {
         S.candidates += sm.cnt_cand;
         S.pair_evals += sm.cnt_eval + sm.A.nbstat + (sm.A.hbbond > 0 ? 1 : 0);
     }
+    PROF_MARK(S, PH_TAIL, pt);
     // statistics: candidates scanned, quadratic evaluations
     if (err) atomicCAS(&sm.err, 0, err);
     __syncthreads();
@@ -1113,11 +1160,52 @@ __device__ unsigned long long state_hash(const Sys &S) {
     return h;
 }
 
+// UpdateData + TimeForward over all atoms, ToolFunctions.c:23-36
+template <int B>
+__device__ __forceinline__ void sweep(const Sys &S, const double dt) {
+    double *__restrict__ X = S.x;
+    double *__restrict__ Y = S.y;
+    double *__restrict__ Z = S.z;
+    double *__restrict__ T = S.evt;
+    const double *__restrict__ VX = S.vx;
+    const double *__restrict__ VY = S.vy;
+    const double *__restrict__ VZ = S.vz;
+    const int N = S.N;
+#pragma unroll 4
+    for (int i = threadIdx.x + 1; i <= N; i += B) {
+        const double vx = VX[i], vy = VY[i], vz = VZ[i];
+        const double x = X[i], y = Y[i], z = Z[i], t = T[i];
+        X[i] = x + vx * dt;
+        Y[i] = y + vy * dt;
+        Z[i] = z + vz * dt;
+        T[i] = t - dt;
+    }
+}
+
 // ---------------------------------------------------------------- kernels
+// The system struct is copied into LDS for the kernel's lifetime: its
+// pointers and constants are then LDS reads instead of dependent global
+// loads, and the run scalars it holds are written back at the end.
+template <int B>
+__device__ __forceinline__ void sys_load(Sys &dst, const Sys *src) {
+    const int n = (int)(sizeof(Sys) / sizeof(int));
+    for (int k = threadIdx.x; k < n; k += B) ((int *)&dst)[k] = ((const int *)src)[k];
+    __syncthreads();
+}
+
+template <int B>
+__device__ __forceinline__ void sys_store(Sys *dst, const Sys &src) {
+    __syncthreads();
+    const int n = (int)(sizeof(Sys) / sizeof(int));
+    for (int k = threadIdx.x; k < n; k += B) ((int *)dst)[k] = ((const int *)&src)[k];
+}
+
 template <int B>
 __global__ __launch_bounds__(B) void k_prepare(Sys *systems) {
     __shared__ Smem sm;
-    Sys &S = systems[blockIdx.x];
+    __shared__ Sys Sl;
+    sys_load<B>(Sl, &systems[blockIdx.x]);
+    Sys &S = Sl;
     const int tid = threadIdx.x;
     if (tid == 0) { sm.err = 0; sm.zpos = S.zpos; }
     __syncthreads();
@@ -1134,16 +1222,20 @@ __global__ __launch_bounds__(B) void k_prepare(Sys *systems) {
         S.zpos = sm.zpos;
         if (sm.err) S.error = sm.err;
     }
+    sys_store<B>(&systems[blockIdx.x], Sl);
 }
 
 template <int B>
 __global__ __launch_bounds__(B) void k_run(Sys *systems, long long npops) {
     __shared__ Smem sm;
-    Sys &S = systems[blockIdx.x];
+    __shared__ Sys Sl;
+    sys_load<B>(Sl, &systems[blockIdx.x]);
+    Sys &S = Sl;
     const int tid = threadIdx.x;
     if (tid == 0) { sm.err = S.error; sm.zpos = S.zpos; }
     __syncthreads();
     const int N = S.N;
+    unsigned long long pt = S.prof ? wall_clock64() : 0;
     for (long long pop = 0; pop < npops; pop++) {
         if (sm.err) break;
         if (tid == 0) {
@@ -1166,17 +1258,13 @@ __global__ __launch_bounds__(B) void k_run(Sys *systems, long long npops) {
         }
         __syncthreads();
         if (sm.err) break;
+        PROF_MARK(S, PH_POP, pt);
         // UpdateData + TimeForward, ToolFunctions.c:23-36
-        const double dt = sm.dt;
-        for (int i = tid + 1; i <= N; i += B) {
-            S.x[i] += S.vx[i] * dt;
-            S.y[i] += S.vy[i] * dt;
-            S.z[i] += S.vz[i] * dt;
-            S.evt[i] -= dt;
-        }
+        sweep<B>(S, sm.dt);
         __syncthreads();
+        PROF_MARK(S, PH_SWEEP, pt);
         if (tid == 0) {
-            S.currenttime += dt;
+            S.currenttime += sm.dt;
             const int target = sm.target, partner = sm.partner;
             // HazardCheck, SGThread.c:215-235
             sm.stale = (partner > 0 && S.evpc[target] < S.evc[partner]);
@@ -1194,9 +1282,12 @@ __global__ __launch_bounds__(B) void k_run(Sys *systems, long long npops) {
         }
         __syncthreads();
         if (sm.err) break;
+        PROF_MARK(S, PH_EVENT, pt);
         predict_list<B>(S, sm, sm.renew);
+        if (S.prof) pt = wall_clock64();
         if (tid == 0) {
             for (int k = 1; sm.renew[k]; k++) cbt_update_run(S, sm.renew[k]);
+            PROF_MARK(S, PH_TREE, pt);
             S.pops++;
             if (S.trace && S.trace_n < S.trace_cap) {
                 sdmd_gpu_pop_rec &r = S.trace[S.trace_n];
@@ -1224,6 +1315,7 @@ __global__ __launch_bounds__(B) void k_run(Sys *systems, long long npops) {
         S.zpos = sm.zpos;
         if (sm.err) S.error = sm.err;
     }
+    sys_store<B>(&systems[blockIdx.x], Sl);
 }
 
 // single operations behind the fine-grained entry points
@@ -1232,7 +1324,9 @@ enum { OP_PREDICT, OP_CREATE_TREE, OP_UPDATE_TREE, OP_ADVANCE, OP_DO_EVENT, OP_C
 template <int B>
 __global__ __launch_bounds__(B) void k_op(Sys *systems, int sys, int op, const int *renew, double dt, int a, int b) {
     __shared__ Smem sm;
-    Sys &S = systems[sys];
+    __shared__ Sys Sl;
+    sys_load<B>(Sl, &systems[sys]);
+    Sys &S = Sl;
     const int tid = threadIdx.x;
     if (tid == 0) { sm.err = 0; sm.zpos = S.zpos; }
     __syncthreads();
@@ -1265,6 +1359,7 @@ __global__ __launch_bounds__(B) void k_op(Sys *systems, int sys, int op, const i
         S.zpos = sm.zpos;
         if (sm.err) S.error = sm.err;
     }
+    sys_store<B>(&systems[sys], Sl);
 }
 
 }  // namespace
@@ -1322,6 +1417,18 @@ static int upload_system(sdmd_gpu_ctx *c, Sys *S, const sdmd_system_desc *d, con
     S->thermostat = d->thermostat;
     for (int k = 0; k < 6; k++) S->hbpot[k] = d->hb_pot[k];
     S->currenttime = d->currenttime;
+    S->steps_sorted = 1;
+    {
+        auto check_list = [&](const sdmd_pair &p) {
+            for (int k = 0; k < p.nstep; k++) {
+                const double v = d->step_d[p.step0 + k];
+                if (!(v > 0) || (k > 0 && !(v > d->step_d[p.step0 + k - 1]))) S->steps_sorted = 0;
+            }
+        };
+        for (int k = 0; k < NT * NT; k++) check_list(d->pair[k]);
+        for (int k = 0; k < SDMD_NHBCLASS * NT * NT; k++) check_list(d->hbpair[k]);
+        for (int k = 0; k < d->con_off[d->natom + 1]; k++) check_list(d->con[k]);
+    }
     int rc = 0;
 #define UP(field, src, n) if ((rc = dev_upload(c, &S->field, src, n))) return rc
     UP(x, d->x, N1); UP(y, d->y, N1); UP(z, d->z, N1);
@@ -1350,6 +1457,9 @@ static int upload_system(sdmd_gpu_ctx *c, Sys *S, const sdmd_system_desc *d, con
     if (o && o->trace) {
         S->trace_cap = o->trace_capacity > 0 ? o->trace_capacity : 100000;
         UP(trace, (const sdmd_gpu_pop_rec *)nullptr, (size_t)S->trace_cap);
+    }
+    if (o && (o->profile & 1)) {
+        UP(prof, (const unsigned long long *)nullptr, (size_t)PH_NPH);
     }
     if (o && o->hash_every > 0) {
         S->hash_every = o->hash_every;
@@ -1617,6 +1727,21 @@ extern "C" int sdmd_gpu_get_hashes(sdmd_gpu_ctx *c, int32_t s, int64_t *out, int
     if (k > 0 && out && hipMemcpy(out, S.hashes, sizeof(int64_t) * 2 * (size_t)k, hipMemcpyDeviceToHost) != hipSuccess)
         return SDMD_EDEVICE;
     *n = k;
+    return SDMD_OK;
+}
+
+// per-phase device time (seconds) of system s when opened with profiling on
+extern "C" int sdmd_gpu_phase_times(sdmd_gpu_ctx *c, int32_t s, double *out, int32_t n) {
+    if (!c || s < 0 || s >= c->nsys || !out) return SDMD_EINVAL;
+    const Sys &S = c->host_sys[s];
+    if (!S.prof) return SDMD_ESTATE;
+    unsigned long long v[PH_NPH];
+    if (hipMemcpy(v, S.prof, sizeof v, hipMemcpyDeviceToHost) != hipSuccess) return SDMD_EDEVICE;
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+        return SDMD_EDEVICE;
+    for (int k = 0; k < n && k < PH_NPH; k++) out[k] = (double)v[k] / (1e3 * khz);
     return SDMD_OK;
 }
 

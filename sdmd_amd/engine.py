@@ -98,13 +98,15 @@ class System:
 class GpuEngine:
     """Device-resident systems on one HIP stream (one workgroup per system)."""
 
+    PHASES = ["pop", "sweep", "event", "stage", "gather", "bondsel", "eval", "select", "hb", "tail", "tree"]
+
     def __init__(self, systems, block=256, trace=False, trace_capacity=0, hash_every=0,
-                 normals_per_pop=5.0, max_pops=0):
+                 normals_per_pop=5.0, max_pops=0, profile=False):
         self.systems = list(systems)
         L = _lib.lib()
         self._descs = [s.desc() for s in self.systems]
         arr = (C.POINTER(SystemDesc) * len(self._descs))(*[C.pointer(d) for d in self._descs])
-        opts = GpuOpts(0, block, 1 if trace else 0, 0, trace_capacity, hash_every)
+        opts = GpuOpts(0, block, 1 if trace else 0, 1 if profile else 0, trace_capacity, hash_every)
         h = C.c_void_p()
         check(L.sdmd_gpu_open(arr, len(self._descs), C.byref(opts), C.byref(h)), "sdmd_gpu_open")
         self._h = h
@@ -129,6 +131,11 @@ class GpuEngine:
 
     def run(self, pops):
         check(_lib.lib().sdmd_gpu_run(self._h, int(pops)), "sdmd_gpu_run")
+
+    def phase_times(self, sys=0):
+        out = np.zeros(len(self.PHASES))
+        check(_lib.lib().sdmd_gpu_phase_times(self._h, sys, out.ctypes.data, len(out)), "sdmd_gpu_phase_times")
+        return dict(zip(self.PHASES, out))
 
     def last_kernel_ms(self):
         ms = C.c_double()
