@@ -31,6 +31,10 @@
  *                         (AssignJob, UpdateData, TimeForward, HazardCheck
  *                         215-235, DoEvent, Predict, UpdateCBT), run on the
  *                         device for a given number of pops per system
+ * In SDMD_MODE_FAST only open/prepare/run/run_for/download/stats apply; the
+ * single-step entry points (predict, create/update_tree, pop, advance,
+ * do_event, cell_move, get_tree, traces) return SDMD_EUNSUPPORTED, as they
+ * address the reference's swept, relative-time state.
  * renew lists follow the reference's convention: renew[0] = count, atom ids
  * from renew[1], zero-terminated.
  */
@@ -46,15 +50,24 @@ extern "C" {
 
 typedef struct sdmd_gpu_ctx sdmd_gpu_ctx;
 
-enum { SDMD_MODE_PLAIN = 0 };
+enum {
+    SDMD_MODE_PLAIN = 0,   /* FP64 on the reference's swept state: bit-exact with the reference */
+    SDMD_MODE_FAST = 1     /* local-time state, FP32 pair-time screen + FP64 confirm, concurrent
+                              independent events (same event semantics, own trajectory) */
+};
 
 typedef struct {
-    int32_t mode;          /* SDMD_MODE_PLAIN: FP64, bit-exact with the reference */
-    int32_t block;         /* threads per system workgroup (64..1024, multiple of 64); 0 = 256 */
-    int32_t trace;         /* record every pop (parity checks) */
-    int32_t profile;       /* bit 0: accumulate per-phase device time */
+    int32_t mode;          /* SDMD_MODE_PLAIN or SDMD_MODE_FAST */
+    int32_t block;         /* threads per system workgroup: plain 64..1024 (power of two),
+                              fast 64, 256, 512 or 1024 (one event per wavefront); 0 = 256 */
+    int32_t trace;         /* plain: record every pop (parity checks) */
+    int32_t profile;       /* plain, bit 0: accumulate per-phase device time */
     int64_t trace_capacity;
-    int64_t hash_every;    /* state hash every n pops (0 = off) */
+    int64_t hash_every;    /* plain: state hash every n pops (0 = off) */
+    int32_t validate;      /* fast: also solve every collision candidate in FP64 and count
+                              agreement of the screened prediction with the full FP64 scan */
+    int32_t max_batch;     /* fast: at most this many events per round (0 = 64) */
+    uint64_t seed;         /* fast: thermostat stream key (0 = the system's seed) */
 } sdmd_gpu_opts;
 
 /* one popped event (same layout as the reference trace dump) */
@@ -79,6 +92,22 @@ typedef struct {
     int32_t pad;
 } sdmd_gpu_stats;
 
+/* fast-mode counters of one system */
+typedef struct {
+    int64_t batches;             /* rounds of concurrent events */
+    int64_t deferred;            /* earliest-prefix events held back by the independence test */
+    int64_t single;              /* rounds that fell back to the single earliest event */
+    int64_t screens;             /* FP32 pair-time screens */
+    int64_t confirms;            /* FP64 re-solves of screened candidates */
+    int64_t validated;           /* predictions re-checked against a full FP64 scan */
+    int64_t agree;               /* ... with the same partner, subtype and time */
+    int64_t rebases;             /* epoch moves of the FP32 mirror */
+    int64_t causal;              /* reads of an atom whose local time is later than the reader's */
+    double max_screen_rel_err;   /* largest |t32 - t64| / t64 over confirmed collision winners */
+    double max_time_rel_err;     /* largest time error of a disagreeing validated prediction */
+    double clock;                /* time of the latest executed event */
+} sdmd_gpu_fast_stats;
+
 typedef struct {
     int32_t target, partner, type, sub;
     double time, potential;
@@ -98,6 +127,11 @@ int sdmd_gpu_set_normals(sdmd_gpu_ctx *ctx, int32_t sys, const double *z, int64_
 
 int sdmd_gpu_prepare(sdmd_gpu_ctx *ctx);
 int sdmd_gpu_run(sdmd_gpu_ctx *ctx, int64_t pops_per_system);
+/* As sdmd_gpu_run, but every system also stops once max_ms of device time
+ * has passed since the launch (max_ms <= 0: no limit).  Aggregate rates over
+ * uneven systems are then measured over a common wall-clock window. */
+int sdmd_gpu_run_for(sdmd_gpu_ctx *ctx, int64_t max_pops_per_system, double max_ms);
+int sdmd_gpu_fast_stats_get(sdmd_gpu_ctx *ctx, int32_t sys, sdmd_gpu_fast_stats *out);
 
 int sdmd_gpu_predict(sdmd_gpu_ctx *ctx, int32_t sys, const int32_t *renew);
 int sdmd_gpu_create_tree(sdmd_gpu_ctx *ctx, int32_t sys);

@@ -68,6 +68,12 @@ typedef struct { int32_t r[34]; int f, b; int64_t calls; } sdmd_rand_state;
 void sdmd_rand_seed(sdmd_rand_state *st, uint32_t seed);
 int32_t sdmd_rand_next(sdmd_rand_state *st);
 
+/* Total kinetic and potential energy of a flattened state (kcal/mol):
+ * CalKinetE (ToolFunctions.c:626-656, no flow) and CalPotenE
+ * (ToolFunctions.c:659-752: every pair within the cut-off through FindPair's
+ * accumulated shell energy, minus the H-bond barrier per formed H bond). */
+int sdmd_host_energy(const sdmd_system_desc *d, double *kinetic, double *potential);
+
 void sdmd_host_free(sdmd_host_system *s);
 
 /* Human-readable name of an SDMD_E* code. */

@@ -27,7 +27,15 @@ class PopRec(C.Structure):
 
 class GpuOpts(C.Structure):
     _fields_ = [("mode", C.c_int32), ("block", C.c_int32), ("trace", C.c_int32),
-                ("profile", C.c_int32), ("trace_capacity", C.c_int64), ("hash_every", C.c_int64)]
+                ("profile", C.c_int32), ("trace_capacity", C.c_int64), ("hash_every", C.c_int64),
+                ("validate", C.c_int32), ("max_batch", C.c_int32), ("seed", C.c_uint64)]
+
+
+class FastStats(C.Structure):
+    _fields_ = [("batches", C.c_int64), ("deferred", C.c_int64), ("single", C.c_int64),
+                ("screens", C.c_int64), ("confirms", C.c_int64), ("validated", C.c_int64),
+                ("agree", C.c_int64), ("rebases", C.c_int64), ("causal", C.c_int64),
+                ("max_screen_rel_err", C.c_double), ("max_time_rel_err", C.c_double), ("clock", C.c_double)]
 
 
 class GpuStats(C.Structure):
@@ -45,13 +53,14 @@ class GpuEvent(C.Structure):
 
 # every symbol declared in include/sdmd_host.h and include/sdmd_gpu.h
 HOST_SYMBOLS = ["sdmd_host_load", "sdmd_host_prepare", "sdmd_host_desc", "sdmd_host_normals",
-                "sdmd_normal_stream",
+                "sdmd_normal_stream", "sdmd_host_energy",
                 "sdmd_rand_seed", "sdmd_rand_next", "sdmd_host_free", "sdmd_strerror"]
 GPU_SYMBOLS = ["sdmd_gpu_open", "sdmd_gpu_close", "sdmd_gpu_set_normals", "sdmd_gpu_prepare",
                "sdmd_gpu_run", "sdmd_gpu_predict", "sdmd_gpu_create_tree", "sdmd_gpu_update_tree",
                "sdmd_gpu_pop", "sdmd_gpu_advance", "sdmd_gpu_do_event", "sdmd_gpu_cell_move",
                "sdmd_gpu_download", "sdmd_gpu_get_tree", "sdmd_gpu_stats_get", "sdmd_gpu_get_trace",
-               "sdmd_gpu_get_hashes", "sdmd_gpu_phase_times", "sdmd_gpu_last_kernel_ms"]
+               "sdmd_gpu_get_hashes", "sdmd_gpu_phase_times", "sdmd_gpu_last_kernel_ms",
+               "sdmd_gpu_run_for", "sdmd_gpu_fast_stats_get"]
 
 
 def lib():
@@ -69,6 +78,7 @@ def lib():
     L.sdmd_host_desc.argtypes = [P]
     L.sdmd_host_normals.argtypes = [P, C.c_void_p, C.c_long, C.c_void_p]
     L.sdmd_normal_stream.argtypes = [C.c_uint32, C.c_int64, C.c_void_p, C.c_long, C.c_void_p]
+    L.sdmd_host_energy.argtypes = [C.POINTER(SystemDesc), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.sdmd_rand_seed.argtypes = [C.c_void_p, C.c_uint32]
     L.sdmd_rand_next.restype = C.c_int32
     L.sdmd_rand_next.argtypes = [C.c_void_p]
@@ -80,6 +90,8 @@ def lib():
     L.sdmd_gpu_set_normals.argtypes = [P, I, C.c_void_p, I64]
     L.sdmd_gpu_prepare.argtypes = [P]
     L.sdmd_gpu_run.argtypes = [P, I64]
+    L.sdmd_gpu_run_for.argtypes = [P, I64, C.c_double]
+    L.sdmd_gpu_fast_stats_get.argtypes = [P, C.c_int32, C.POINTER(FastStats)]
     L.sdmd_gpu_predict.argtypes = [P, I, C.c_void_p]
     L.sdmd_gpu_create_tree.argtypes = [P, I]
     L.sdmd_gpu_update_tree.argtypes = [P, I, C.c_void_p]

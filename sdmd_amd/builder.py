@@ -371,17 +371,30 @@ def _place(coords, src_a, src_b, src_c, dst_a, dst_b, ref, torsion):
 _PHI_PSI = [(-120.0, 130.0), (-75.0, 145.0), (-65.0, -40.0), (-140.0, 155.0)]
 
 
-def build_chain(seq, rng, extended=False, lib_dir=LIB_DIR):
+def build_chain(seq, rng, extended=False, lib_dir=LIB_DIR, attempts=4):
     """Fragment assembly: residue conformers placed along the chain with
     chosen backbone torsions (all 180 deg for an extended chain, otherwise
     seeded draws from common basins with steric rejection), then polished
-    by the restraint/hard-core minimiser."""
-    from scipy.spatial import cKDTree
-
+    by the restraint/hard-core minimiser.  A polish that leaves violations
+    is retried with fresh torsions (up to `attempts` assemblies) before the
+    slower full distance-geometry build."""
     tpl = templates(lib_dir)
     core = pair_core_table(lib_dir)
     ch = Chain(list(seq))
     ch.names, ch.resid, ch.resname, ch.types = _chain_atoms(seq, tpl)
+    restr = _restraints(seq, ch.names, ch.resid, tpl)
+    for _ in range(1 if extended else attempts):
+        xyz = _assemble(seq, ch, rng, extended, lib_dir)
+        xyz, nviol = _relax(xyz, restr, ch.types, core, rng, rounds=6)
+        if not nviol:
+            ch.xyz = xyz - xyz.mean(axis=0)
+            return ch
+    return build_chain_dg(seq, rng, extended=extended, lib_dir=lib_dir)
+
+
+def _assemble(seq, ch, rng, extended, lib_dir):
+    from scipy.spatial import cKDTree
+
     placed = {}
     prev = None           # (N, CA, C) of previous residue, and unit positions for this one
     for r, aa in enumerate(seq, start=1):
@@ -422,14 +435,7 @@ def build_chain(seq, rng, extended=False, lib_dir=LIB_DIR):
         prev = {"C": cur["C"]}
         if nxt is not None:
             prev.update(nxt)
-    xyz = np.array([placed[(r, n)] for r, n in zip(ch.resid, ch.names)])
-    restr = _restraints(seq, ch.names, ch.resid, tpl)
-    xyz, nviol = _relax(xyz, restr, ch.types, core, rng, rounds=6)
-    if nviol:
-        # fall back to the slower full distance-geometry build
-        return build_chain_dg(seq, rng, extended=extended, lib_dir=lib_dir)
-    ch.xyz = xyz - xyz.mean(axis=0)
-    return ch
+    return np.array([placed[(r, n)] for r, n in zip(ch.resid, ch.names)])
 
 
 def _random_rotation(rng):
@@ -575,6 +581,43 @@ def random_peptides(nchains, length, seed, mM=5.0, box=None, sequences=None, min
     chains = build_chains(sequences, seed, workers=workers)
     place_chains(chains, (box, box, box), np.random.default_rng([seed, 104729]), min_gap=min_gap)
     return System(chains, (box, box, box), "random peptides seed %d" % seed)
+
+
+def batch_specs(nsys=48, seed=4, length=(10, 80), chains=(1, 32), min_atoms=2000,
+                atoms_per_residue=10.45):
+    """(nchains, length) of each system of a seeded batch of uneven peptide
+    systems: chain length and chain count uniform in the given ranges,
+    redrawn until the expected atom count reaches min_atoms."""
+    rng = np.random.default_rng([seed, 31337])
+    out = []
+    while len(out) < nsys:
+        nc = int(rng.integers(chains[0], chains[1] + 1))
+        L = int(rng.integers(length[0], length[1] + 1))
+        if nc * L * atoms_per_residue >= min_atoms:
+            out.append((nc, L))
+    return out
+
+
+def batch_peptides(nsys=48, seed=4, mM=5.0, workers=None, **kw):
+    """A batch of independent peptide systems of uneven size (each its own
+    cubic periodic box at mM per chain, widened where a chain's extent needs
+    it: edge >= 2*extent + 8 A).  Every chain of every system is built in
+    one worker pool; system k uses seed (seed, k)."""
+    specs = batch_specs(nsys, seed, **kw)
+    seqs, owner = [], []
+    for k, (nc, L) in enumerate(specs):
+        s = random_sequences(nc, L, seed * 1000 + k)
+        seqs += s
+        owner += [k] * len(s)
+    chains = build_chains(seqs, seed, workers=workers)
+    systems = []
+    for k, (nc, L) in enumerate(specs):
+        mine = [c for c, o in zip(chains, owner) if o == k]
+        ext = max(np.max(np.linalg.norm(c.xyz, axis=1)) for c in mine)
+        box = max(box_for_concentration(nc, mM), 2 * ext + 8.0)
+        place_chains(mine, (box, box, box), np.random.default_rng([seed, k, 104729]))
+        systems.append(System(mine, (box, box, box), "batch seed %d system %d" % (seed, k)))
+    return systems
 
 
 def write_input_dir(system: System, outdir: str, gro_name="coordinate.gro", **param):

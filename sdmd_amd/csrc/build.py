@@ -19,7 +19,7 @@ LIB = os.path.join(PKG, "libsdmd.so")
 ARCH = os.environ.get("SDMD_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["host/sdmd_host.c"]
-HIP_SOURCES = ["sdmd_gpu.hip"]
+HIP_SOURCES = ["sdmd_gpu.hip", "sdmd_fast.hip"]
 
 
 def _hipcc():

@@ -1128,6 +1128,129 @@ int sdmd_normal_stream(uint32_t seed, int64_t skip, double *out, long n, int64_t
 }
 
 /* ------------------------------------------------------------------ */
+/* energy: CalKinetE / CalPotenE                                       */
+/* ------------------------------------------------------------------ */
+static int e_type_change(int t, int direct) {
+    if (direct == 1) { if (t == 1) return 3; if (t == 2) return 4; if (t == 26) return 27; }
+    else { if (t == 3) return 1; if (t == 4) return 2; if (t == 27) return 26; }
+    return t;
+}
+
+static const sdmd_pair *e_right_pair(const sdmd_system_desc *d, int t1, int t2) {
+    const sdmd_pair *p = &d->pair[t1 * SDMD_NTYPE + t2];
+    if (p->dmin > 0) return p;
+    int a = e_type_change(t1, 0), b = e_type_change(t2, 0);
+    if ((p = &d->pair[a * SDMD_NTYPE + t2])->dmin > 0) return p;
+    if ((p = &d->pair[t1 * SDMD_NTYPE + b])->dmin > 0) return p;
+    if ((p = &d->pair[a * SDMD_NTYPE + b])->dmin > 0) return p;
+    return NULL;
+}
+
+static int e_hb_model(const sdmd_system_desc *d, int a1, int a2) {   /* HBModel, Models.c:212-290 */
+    if (d->hb_role[a1] == 'A' || d->hb_role[a1] == 'S') { int t = a1; a1 = a2; a2 = t; }
+    int o1 = e_type_change(d->type[a1], 0), o2 = e_type_change(d->type[a2], 0);
+    if (o1 == 2 && o2 == 26) return 1;
+    if (o1 == 2) {
+        switch (d->aatype[a2]) {
+        case 4: case 7: return 2; case 16: case 17: case 19: return 3; case 9: return 4;
+        case 5: return 10; case 3: case 6: return -1; default: break;
+        }
+    } else if (o2 == 26) return 5;
+    else if (o1 == 31 && o2 == 31) return 9;
+    else if (o1 == 1) {
+        switch (d->aatype[a2]) {
+        case 4: case 7: return 6; case 16: case 17: case 19: return 7; case 9: return 8;
+        case 3: case 6: return 11; default: break;
+        }
+    }
+    return -1;
+}
+
+/* accumulated energy of FindPair (ToolFunctions.c:39-224) */
+static double e_accum(const sdmd_system_desc *d, const sdmd_pair *c, double d2, double direction) {
+    if (!c || c->dmin == 0) return SDMD_INFINIT;
+    double tol = (direction < 0 ? 1 : -1) * SDMD_ZERO;
+    if (d2 < c->dmin - SDMD_ZERO) return SDMD_INFINIT;
+    for (int k = 0; k < c->nstep && d->step_d[c->step0 + k] != 0; k++)
+        if (d2 < d->step_d[c->step0 + k] + tol) return d->step_acc[c->step0 + k];
+    if (c->dmax > 0) return d2 < c->dmax + SDMD_ZERO ? 0 : SDMD_INFINIT;
+    return SDMD_INFINIT;
+}
+
+static int e_cmap(const sdmd_system_desc *d, int i, int j) {
+    int bits = 0;
+    for (int k = d->cmap_off[i]; k < d->cmap_off[i + 1]; k++)
+        if (d->cmap_j[k] == j) { bits = d->cmap_bits[k]; break; }
+    for (int k = 0; k < SDMD_DYNSLOTS; k++)
+        if (d->dyn_j[i * SDMD_DYNSLOTS + k] == j) { bits |= d->dyn_bits[i * SDMD_DYNSLOTS + k]; break; }
+    return bits;
+}
+
+int sdmd_host_energy(const sdmd_system_desc *d, double *kinetic, double *potential) {
+    if (!d || !kinetic || !potential) return SDMD_EINVAL;
+    static const int OX[27] = {0, 1, 0, 0, 0, 1, 1, 1, -1, 0, 0, 0, -1, -1, -1, 0, 0, 1, -1, 1, -1, -1, -1, 1, 1, 1, -1};
+    static const int OY[27] = {0, 0, 1, 0, 1, 0, 1, 1, 0, -1, 0, -1, 0, -1, -1, -1, 1, 0, 0, -1, 1, -1, 1, -1, 1, -1, 1};
+    static const int OZ[27] = {0, 0, 0, 1, 1, 1, 0, 1, 0, 0, -1, -1, -1, 0, -1, 1, -1, -1, 1, 0, 0, 1, -1, -1, -1, 1, 1};
+    const int N = d->natom;
+    double ke = 0;
+    for (int i = 1; i <= N; i++) {
+        double v2 = d->vx[i] * d->vx[i] + d->vy[i] * d->vy[i] + d->vz[i] * d->vz[i];
+        ke += 0.5 * d->mass[i] * v2;
+    }
+    double pe = 0, cut2 = d->cutoffr * d->cutoffr;
+    const int *cn = d->cellnum;
+    for (int i = 1; i <= N; i++) {
+        int ci[3] = {d->cellx[i], d->celly[i], d->cellz[i]};
+        for (int n = 0; n < 27; n++) {
+            int c[3] = {ci[0] + OX[n], ci[1] + OY[n], ci[2] + OZ[n]};
+            double sh[3] = {0, 0, 0};
+            for (int k = 0; k < 3; k++) {
+                if (c[k] < 0) { c[k] = cn[k] - 1; sh[k] = -d->box[k]; }
+                else if (c[k] >= cn[k]) { c[k] = 0; sh[k] = d->box[k]; }
+            }
+            int cell = c[2] * cn[0] * cn[1] + c[1] * cn[0] + c[0] + 1;
+            for (int j = d->celllist[N + cell]; j; j = d->celllist[j]) {
+                if (j <= i) continue;
+                double r[3] = {d->x[i] - (d->x[j] + sh[0]), d->y[i] - (d->y[j] + sh[1]), d->z[i] - (d->z[j] + sh[2])};
+                double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                int bits = e_cmap(d, i, j);
+                if (r2 > cut2 || (bits & SDMD_BOND_CONNECT)) continue;
+                double v[3] = {d->vx[i] - d->vx[j], d->vy[i] - d->vy[j], d->vz[i] - d->vz[j]};
+                double dir = r[0] * v[0] + r[1] * v[1] + r[2] * v[2];
+                const sdmd_pair *p = NULL;
+                int a = i, b = j;
+                if (bits & SDMD_HB_CONNECT) {
+                    int m = e_hb_model(d, i, j);
+                    p = m >= 0 ? &d->hbpair[(m * SDMD_NTYPE) * SDMD_NTYPE] : NULL;
+                    /* HBBarrier, Events.c:1097-1119 */
+                    int a1 = i, a2 = j;
+                    if (d->hb_role[a1] == 'A' || d->hb_role[a1] == 'S') { a1 = j; a2 = i; }
+                    int o1 = e_type_change(d->type[a1], 0), o2 = e_type_change(d->type[a2], 0);
+                    pe -= (o1 == 2 && o2 == 26) ? d->hb_pot[1] : (o1 != 2 && o2 != 26) ? d->hb_pot[5] : d->hb_pot[3];
+                } else if (bits & SDMD_CONSTRAINT_CONNECT) {
+                    for (int k = d->con_off[i]; k < d->con_off[i + 1]; k++)
+                        if (d->con_j[k] == j) { p = &d->con[k]; break; }
+                } else if (bits & SDMD_NEIGHBOR_CONNECT) {
+                    if (d->hb_neighbor[d->hb_bond[a]] == b) { int t = a; a = b; b = t; }
+                    int hbp = d->hb_bond[b];
+                    int m = e_hb_model(d, b, hbp);
+                    if (m >= 0)
+                        p = &d->hbpair[(m * SDMD_NTYPE + e_type_change(d->type[a], 0)) * SDMD_NTYPE +
+                                       e_type_change(d->type[b], 0)];
+                } else {
+                    p = e_right_pair(d, d->type[i], d->type[j]);
+                }
+                double acc = e_accum(d, p, r2, dir);
+                if (acc < SDMD_INFINIT / 2) pe += acc;
+            }
+        }
+    }
+    *kinetic = ke;
+    *potential = pe;
+    return SDMD_OK;
+}
+
+/* ------------------------------------------------------------------ */
 /* flattening                                                         */
 /* ------------------------------------------------------------------ */
 static void *flat_alloc(sdmd_host_system *s, size_t n) {

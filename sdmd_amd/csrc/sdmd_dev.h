@@ -1,0 +1,586 @@
+// sdmd_dev.h - device data structures and helpers shared by the plain
+// FP64 engine (sdmd_gpu.hip) and the screened local-time engine
+// (sdmd_fast.hip): the resident system struct, the force-field lookups
+// (RightPair, HBModel, HBBarrier, FindPair), the contact-time quadratic
+// and the connection map.  Compiled with -ffp-contract=off.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#define HIPV(x) ((void)(x))
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "sdmd_gpu.h"
+#include "sdmd_host.h"
+
+namespace sdmd {
+
+constexpr double INFINIT = SDMD_INFINIT;
+constexpr double ZERO = SDMD_ZERO;
+constexpr int INVALID = SDMD_INVALID;
+constexpr int NT = SDMD_NTYPE;
+constexpr int CELL_CAP = 64;               // atoms gathered per neighbour cell
+constexpr int MAXC = 27 * CELL_CAP;        // candidates per predicted atom
+constexpr int MAXBOND = 16;                // BondTime's connect[16], TimePrediction.c:188
+constexpr int ROWCAP = 96;                 // connection-map row staged in LDS
+
+enum { K_COLLISION, K_CONSTRAINT, K_HB, K_NEIGHBOR };
+enum { ERR_NONE = 0, ERR_TABLE = 1, ERR_HBMODEL = 2, ERR_SUBTYPE = 3, ERR_PHI_NAN = 4,
+       ERR_DYNBITS = 5, ERR_NBLIST = 6, ERR_NBOVERFLOW = 7, ERR_CELL = 8, ERR_LINKLIST = 9,
+       ERR_EVENTTYPE = 10, ERR_CELLCAP = 11, ERR_NORMALS = 12, ERR_TRACE = 13 };
+
+static __constant__ int c_OX[27] = {0, 1, 0, 0, 0, 1, 1, 1, -1, 0, 0, 0, -1, -1, -1, 0, 0, 1, -1, 1, -1, -1, -1, 1, 1, 1, -1};
+static __constant__ int c_OY[27] = {0, 0, 1, 0, 1, 0, 1, 1, 0, -1, 0, -1, 0, -1, -1, -1, 1, 0, 0, -1, 1, -1, 1, -1, 1, -1, 1};
+static __constant__ int c_OZ[27] = {0, 0, 0, 1, 1, 1, 0, 1, 0, 0, -1, -1, -1, 0, -1, 1, -1, -1, 1, 0, 0, 1, -1, -1, -1, 1, 1};
+
+// One system resident in device memory.
+struct Sys {
+    int N, ncelllist;
+    int cellnum[4];
+    double cellsize[4], box[4];
+    double cutoffr, T, thermoF;
+    int thermostat;
+    double *x, *y, *z, *vx, *vy, *vz, *mass;
+    int *type, *aatype, *protein, *aanum, *cx, *cy, *cz;
+    signed char *role;
+    int *hbnb, *hbbond, *hbitype, *nbp, *nbnum, *nbstat, *dynj, *dynb;
+    int *evp, *evtype, *evsub;
+    long long *evc, *evpc;
+    double *evt, *evpot;
+    int *celllist;
+    int *bond_off, *bond_j;
+    double *bond_dmin, *bond_dmax;
+    int *con_off, *con_j;
+    sdmd_pair *con;
+    int *cmap_off, *cmap_j, *cmap_bits;
+    sdmd_pair *pair, *hbpair;
+    double *sd, *se, *sacc;
+    double hbpot[6];
+    int *node, *leaf;
+    int count;
+    // thermostat normal stream
+    double *zn;
+    long long nz, zpos;
+    // run state
+    double currenttime;
+    long long frame, recal, pops, pair_evals, candidates;
+    long long per_type[16];
+    int error;
+    // trace
+    sdmd_gpu_pop_rec *trace;
+    long long trace_cap, trace_n;
+    long long *hashes;
+    long long hash_every, hash_cap, hash_n;
+    unsigned long long *prof;   // per-phase wall-clock ticks, or null
+    int steps_sorted;           // every step list strictly increasing and non-zero
+};
+
+// phases timed when a context is opened with profiling on
+enum { PH_POP, PH_SWEEP, PH_EVENT, PH_STAGE, PH_GATHER, PH_BONDSEL, PH_EVAL, PH_SELECT, PH_HB, PH_TAIL,
+       PH_TREE, PH_NPH };
+
+#define PROF_MARK(S, ph, t) do { if ((S).prof && threadIdx.x == 0) { unsigned long long _n = wall_clock64(); \
+    (S).prof[ph] += _n - (t); (t) = _n; } } while (0)
+
+// ---------------------------------------------------------------- models
+static __device__ __forceinline__ int type_change(int t, int direct) {   // AtomTypeChange, ToolFunctions.c:233-270
+    if (direct == 1) {
+        if (t == 1) return 3;
+        if (t == 2) return 4;
+        if (t == 26) return 27;
+    } else {
+        if (t == 3) return 1;
+        if (t == 4) return 2;
+        if (t == 27) return 26;
+    }
+    return t;
+}
+
+static __device__ const sdmd_pair *right_pair(const Sys &S, int t1, int t2) {   // RightPair, ToolFunctions.c:278-292
+    const sdmd_pair *p = &S.pair[t1 * NT + t2];
+    if (p->dmin > 0) return p;
+    int a = type_change(t1, 0), b = type_change(t2, 0);
+    p = &S.pair[a * NT + t2];
+    if (p->dmin > 0) return p;
+    p = &S.pair[t1 * NT + b];
+    if (p->dmin > 0) return p;
+    p = &S.pair[a * NT + b];
+    if (p->dmin > 0) return p;
+    return nullptr;
+}
+
+static __device__ int hb_model(const Sys &S, int a1, int a2, int *err) {   // HBModel, Models.c:212-290
+    if (S.role[a1] == 'A' || S.role[a1] == 'S') { int t = a1; a1 = a2; a2 = t; }
+    int o1 = type_change(S.type[a1], 0), o2 = type_change(S.type[a2], 0);
+    if (o1 == 2 && o2 == 26) return 1;
+    if (o1 == 2) {
+        switch (S.aatype[a2]) {
+        case 4: case 7: return 2;
+        case 16: case 17: case 19: return 3;
+        case 9: return 4;
+        case 5: return 10;
+        case 3: case 6: return -1;
+        default: break;
+        }
+    } else if (o2 == 26) {
+        return 5;
+    } else if (o1 == 31 && o2 == 31) {
+        return 9;
+    } else if (o1 == 1) {
+        switch (S.aatype[a2]) {
+        case 4: case 7: return 6;
+        case 16: case 17: case 19: return 7;
+        case 9: return 8;
+        case 3: case 6: return 11;
+        default: break;
+        }
+    }
+    *err = ERR_HBMODEL;
+    return 0;
+}
+
+static __device__ double hb_barrier(const Sys &S, int a1, int a2) {   // HBBarrier, Events.c:1097-1119
+    if (S.role[a1] == 'A' || S.role[a1] == 'S') { int t = a1; a1 = a2; a2 = t; }
+    int o1 = type_change(S.type[a1], 0), o2 = type_change(S.type[a2], 0);
+    if (o1 == 2 && o2 == 26) return S.hbpot[1];
+    if (o1 != 2 && o2 != 26) return S.hbpot[5];
+    return S.hbpot[3];
+}
+
+// ---------------------------------------------------------------- FindPair
+struct FP { double lo, hi, lopot, hipot, acc; };
+
+// FindPair, ToolFunctions.c:39-224, on the flattened tables
+static __device__ int find_pair(const Sys &S, int a1, int a2, int kind, double direction, double d2, FP &f,
+                         int hbpartner, int typechange, int *err) {
+    const sdmd_pair *c = nullptr;
+    int cs = -1;
+    direction = (direction < 0) ? 1 : -1;
+    double tol = direction * ZERO;
+    if (kind == K_COLLISION) {
+        int t1 = S.type[a1];
+        if (typechange == 1) t1 = type_change(t1, 1);
+        else if (typechange == -1) t1 = type_change(t1, 0);
+        c = right_pair(S, t1, S.type[a2]);
+    } else if (kind == K_CONSTRAINT) {
+        for (int k = S.con_off[a1]; k < S.con_off[a1 + 1]; k++)
+            if (S.con_j[k] == a2) { c = &S.con[k]; break; }
+    } else if (kind == K_HB) {
+        int t = hb_model(S, a1, a2, err);
+        c = &S.hbpair[(t * NT + 0) * NT + 0];
+    } else {
+        int t = hb_model(S, a2, hbpartner, err);
+        int t1 = type_change(S.type[a1], 0), t2 = type_change(S.type[a2], 0);
+        c = &S.hbpair[(t * NT + t1) * NT + t2];
+    }
+    if (!c || c->dmin == 0) {
+        *err = ERR_TABLE;
+        f.lo = f.hi = f.acc = INFINIT;
+        f.lopot = f.hipot = 0;
+        return -1;
+    }
+    const double dmin = c->dmin, dmax = c->dmax;
+    const int step0 = c->step0, nstep = c->nstep;
+    const int flag = (nstep == 0);
+    if (d2 < dmin - ZERO) {
+        f.lo = INFINIT; f.hi = INFINIT; f.acc = INFINIT; f.lopot = 0; f.hipot = 0;
+        return cs;
+    }
+    f.lo = dmin;
+    f.lopot = 0;
+    if (S.steps_sorted && nstep <= 16) {
+        // Every step list is strictly increasing and non-zero (checked at
+        // upload), so the walk below stops at the first k with
+        // d2 < d_k + tol, i.e. at the number of radii d2 has passed.  The
+        // radii are read with independent loads.
+        int ks = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            if (k < nstep) ks += (d2 >= S.sd[step0 + k] + tol) ? 1 : 0;
+        if (ks > 0) {
+            f.lo = S.sd[step0 + ks - 1];
+            f.lopot = S.se[step0 + ks - 1];
+        }
+        if (ks < nstep) {
+            const int k = ks;
+            f.hi = S.sd[step0 + k];
+            f.hipot = -1 * S.se[step0 + k];
+            f.acc = S.sacc[step0 + k];
+            if (k == 0) return 0;
+            if (kind == K_HB && k == nstep - 1 && S.hbbond[a1] > 0) return 1;
+            if (kind == K_NEIGHBOR && k == nstep - 1 && S.hbbond[a2] > 0) { f.hipot = 0; return 1; }
+            return cs;
+        }
+    } else
+    for (int k = 0; k < nstep; k++) {
+        const double sd = S.sd[step0 + k];
+        if (sd == 0) break;
+        if (d2 < sd + tol) {
+            f.hi = sd;
+            f.hipot = -1 * S.se[step0 + k];
+            f.acc = S.sacc[step0 + k];
+            if (k == 0) return 0;
+            if (kind == K_HB && k == nstep - 1 && S.hbbond[a1] > 0) return 1;
+            if (kind == K_NEIGHBOR && k == nstep - 1 && S.hbbond[a2] > 0) { f.hipot = 0; return 1; }
+            return cs;
+        }
+        f.lo = sd;
+        f.lopot = S.se[step0 + k];
+    }
+    if (dmax > 0) {
+        if (d2 < dmax + ZERO) {
+            f.hi = dmax; f.hipot = 0; f.acc = 0;
+            cs = 1;
+            if (flag == 1 && direction == 1) cs = 0;
+            return cs;
+        }
+        f.lo = INFINIT; f.hi = INFINIT; f.acc = INFINIT; f.lopot = 0; f.hipot = 0;
+    } else {
+        f.hi = INFINIT; f.hipot = INFINIT; f.acc = INFINIT;
+        return 1;
+    }
+    return cs;
+}
+
+// ---------------------------------------------------------------- kinematics
+struct Kin { double r1, r2, r3, b, v2, rr; };
+
+// PBCShift (ToolFunctions.c:801-817) + CalculateParameters (TimePrediction.c:1081-1090)
+// with atom j's position and velocity given
+static __device__ __forceinline__ void calc_params_at(const Sys &S, double xi, double yi, double zi, double vxi,
+                                                      double vyi, double vzi, int cxi, int cyi, int czi, int j,
+                                                      double xj, double yj, double zj, double vxj, double vyj,
+                                                      double vzj, Kin &k) {
+    const int cxj = S.cx[j], cyj = S.cy[j], czj = S.cz[j];
+    double s1 = 0, s2 = 0, s3 = 0;
+    if (cxi != cxj) {
+        if (cxi == 0 && cxj == S.cellnum[1] - 1) s1 = -1 * S.box[1];
+        else if (cxi == S.cellnum[1] - 1 && cxj == 0) s1 = S.box[1];
+    }
+    if (cyi != cyj) {
+        if (cyi == 0 && cyj == S.cellnum[2] - 1) s2 = -1 * S.box[2];
+        else if (cyi == S.cellnum[2] - 1 && cyj == 0) s2 = S.box[2];
+    }
+    if (czi != czj) {
+        if (czi == 0 && czj == S.cellnum[3] - 1) s3 = -1 * S.box[3];
+        else if (czi == S.cellnum[3] - 1 && czj == 0) s3 = S.box[3];
+    }
+    const double t1 = xj + s1, t2 = yj + s2, t3 = zj + s3;
+    k.r1 = xi - t1;
+    k.r2 = yi - t2;
+    k.r3 = zi - t3;
+    const double v1 = vxi - vxj, v2 = vyi - vyj, v3 = vzi - vzj;
+    k.rr = k.r1 * k.r1 + k.r2 * k.r2 + k.r3 * k.r3;
+    k.b = k.r1 * v1 + k.r2 * v2 + k.r3 * v3;
+    k.v2 = v1 * v1 + v2 * v2 + v3 * v3;
+}
+
+// the same with j's stored position and velocity
+static __device__ __forceinline__ void calc_params(const Sys &S, double xi, double yi, double zi, double vxi,
+                                                   double vyi, double vzi, int cxi, int cyi, int czi, int j,
+                                                   Kin &k) {
+    calc_params_at(S, xi, yi, zi, vxi, vyi, vzi, cxi, cyi, czi, j, S.x[j], S.y[j], S.z[j], S.vx[j], S.vy[j],
+                   S.vz[j], k);
+}
+
+static __device__ __forceinline__ double calc_disc(const Kin &k, double shortlimit2) {   // CalculateDisc
+    return k.b * k.b - k.v2 * (k.rr - shortlimit2);
+}
+
+static __device__ __forceinline__ double calc_time(const Kin &k, int s, double d2) {      // CalculateTime
+    return (-1 * k.b + s * sqrt(k.b * k.b - k.v2 * (k.rr - d2))) / k.v2;
+}
+
+static __device__ __forceinline__ int cell_index(const Sys &S, int cx, int cy, int cz, int c) {
+    int n1 = cx + c_OX[c], n2 = cy + c_OY[c], n3 = cz + c_OZ[c];
+    if (n1 < 0) n1 = S.cellnum[1] - 1; else if (n1 >= S.cellnum[1]) n1 = 0;
+    if (n2 < 0) n2 = S.cellnum[2] - 1; else if (n2 >= S.cellnum[2]) n2 = 0;
+    if (n3 < 0) n3 = S.cellnum[3] - 1; else if (n3 >= S.cellnum[3]) n3 = 0;
+    return n3 * S.cellnum[1] * S.cellnum[2] + n2 * S.cellnum[1] + n1 + 1;
+}
+
+// static + dynamic connection-map bits of row i (binary search + slots)
+static __device__ int cmap_get(const Sys &S, int i, int j) {
+    int lo = S.cmap_off[i], hi = S.cmap_off[i + 1] - 1, bits = 0;
+    while (lo <= hi) {
+        int m = (lo + hi) >> 1;
+        int v = S.cmap_j[m];
+        if (v == j) { bits = S.cmap_bits[m]; break; }
+        if (v < j) lo = m + 1; else hi = m - 1;
+    }
+    const int *dj = S.dynj + (size_t)i * SDMD_DYNSLOTS;
+    for (int k = 0; k < SDMD_DYNSLOTS; k++)
+        if (dj[k] == j) { bits |= S.dynb[(size_t)i * SDMD_DYNSLOTS + k]; break; }
+    return bits;
+}
+
+static __device__ void cmap_toggle(Sys &S, int i, int j, int bit, int *err) {
+    int *dj = S.dynj + (size_t)i * SDMD_DYNSLOTS, *db = S.dynb + (size_t)i * SDMD_DYNSLOTS;
+    for (int k = 0; k < SDMD_DYNSLOTS; k++)
+        if (dj[k] == j) {
+            db[k] ^= bit;
+            if (db[k] == 0) dj[k] = 0;
+            return;
+        }
+    for (int k = 0; k < SDMD_DYNSLOTS; k++)
+        if (dj[k] == 0) { dj[k] = j; db[k] = bit; return; }
+    *err = ERR_DYNBITS;
+}
+
+
+static __device__ __forceinline__ void wave_argmin(double &t, int &k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        double t2 = __shfl_xor(t, off, 64);
+        int k2 = __shfl_xor(k, off, 64);
+        if (t2 < t || (t2 == t && k2 < k)) { t = t2; k = k2; }
+    }
+}
+
+static __device__ int check_pair_match(const Sys &S, int i, int j, int *err) {   // CheckPairMatch 1119-1138
+    int t = hb_model(S, i, j, err);
+    if (t < 0) return -1;
+    int ti = S.type[i], tj = S.type[j];
+    if (S.hbpair[(t * NT + ti) * NT + tj].nstep == 0) return -1;
+    int ni = type_change(S.type[S.hbnb[i]], 0), nj = type_change(S.type[S.hbnb[j]], 0);
+    if (S.hbpair[(t * NT + ti) * NT + nj].nstep == 0 || S.hbpair[(t * NT + tj) * NT + ni].nstep == 0) return -1;
+    return 0;
+}
+
+static __device__ void list_refresh(int *l, int target) {   // ListRefresh(target, list, 1, 4)
+    int i = 0;
+    for (int k = 1; k < SDMD_NBSLOTS; k++) {
+        if (l[k] == 0) break;
+        if (l[k] == target) { i = k; break; }
+    }
+    if (!i) return;
+    int n = i + 1;
+    while (n <= 4 && l[n] != 0) { l[i] = l[n]; i++; n++; }
+    l[i] = 0;
+}
+
+// ---------------------------------------------------------------- events (lane 0), Events.c
+// Positions and cell membership come from a view: V.pos(j, p) stores atom
+// j's position at the event time (the stored position for the swept engine,
+// x0 + v (t - t0) for the local-time engine); V.first(cell) / V.next(cell, j)
+// walk the atoms of a cell (linked list or cell array).
+struct PosStored {
+    const Sys &S;
+    __device__ void pos(int j, double *p) const { p[0] = S.x[j]; p[1] = S.y[j]; p[2] = S.z[j]; }
+    __device__ int first(int cell) const { return S.celllist[S.N + cell]; }
+    __device__ int next(int, int j) const { return S.celllist[j]; }
+};
+
+template <class P>
+static __device__ void image_near(const Sys &S, const P &V, int i, int j, double *p) {
+    const int ci[3] = {S.cx[i], S.cy[i], S.cz[i]}, cj[3] = {S.cx[j], S.cy[j], S.cz[j]};
+    V.pos(j, p);
+    for (int n = 0; n < 3; n++) {
+        if (ci[n] != cj[n]) {
+            if (ci[n] == 0 && cj[n] == S.cellnum[n + 1] - 1) p[n] -= S.box[n + 1];
+            else if (ci[n] == S.cellnum[n + 1] - 1 && cj[n] == 0) p[n] += S.box[n + 1];
+        }
+    }
+}
+
+// HBEnergyChange, Events.c:1027-1095 (the accumulated value persists across candidates)
+template <class P>
+static __device__ double hb_energy_change(const Sys &S, const P &V, int t, int skip, int hbtype, int *err) {
+    const double cut = S.cutoffr * S.cutoffr;
+    double energy = 0, acc = 0;
+    for (int c = 0; c < 27; c++) {
+        int guard = 0;
+        const int cell = cell_index(S, S.cx[t], S.cy[t], S.cz[t], c);
+        for (int j = V.first(cell); j; j = V.next(cell, j)) {
+            if (++guard > S.N) { *err = ERR_LINKLIST; return energy; }
+            const int bits = cmap_get(S, t, j);
+            if (t != j && !(bits & SDMD_BOND_CONNECT) && !(bits & SDMD_CONSTRAINT_CONNECT) && j != skip) {
+                Kin k;
+                double pt[3], pj[3];
+                V.pos(t, pt);
+                V.pos(j, pj);
+                calc_params_at(S, pt[0], pt[1], pt[2], 0, 0, 0, S.cx[t], S.cy[t], S.cz[t], j, pj[0], pj[1], pj[2],
+                               S.vx[j], S.vy[j], S.vz[j], k);
+                if (k.rr <= cut) {
+                    FP f;
+                    if (hbtype == 0) {
+                        find_pair(S, t, j, K_COLLISION, 0, k.rr, f, 0, -1, err);
+                        acc = f.acc;
+                    } else if (j != S.hbnb[skip]) {
+                        find_pair(S, t, j, K_COLLISION, 0, k.rr, f, 0, 1, err);
+                        acc = f.acc;
+                    }
+                    if (acc == INFINIT) acc = 0;
+                    energy += acc;
+                }
+            }
+        }
+    }
+    return energy;
+}
+
+// InteractionEvent, Events.c:798-1024
+template <class P>
+static __device__ void interaction_event(Sys &S, const P &V, int ti, int pj, int nbi, int nbj, int *err) {
+    double pti[3];
+    V.pos(ti, pti);
+    const double pi0 = pti[0], pi1 = pti[1], pi2 = pti[2];
+    double pjv[3];
+    image_near(S, V, ti, pj, pjv);
+    const double r1 = pi0 - pjv[0], r2 = pi1 - pjv[1], r3 = pi2 - pjv[2];
+    const double v1 = S.vx[ti] - S.vx[pj], v2 = S.vy[ti] - S.vy[pj], v3 = S.vz[ti] - S.vz[pj];
+    const double d2 = r1 * r1 + r2 * r2 + r3 * r3;
+    const double b = r1 * v1 + r2 * v2 + r3 * v3;
+    const double mu = S.mass[ti] * S.mass[pj] / (S.mass[ti] + S.mass[pj]);
+    double pot = S.evpot[ti];
+    int s = 0;
+    if (S.evtype[ti] == SDMD_HB && (S.evsub[ti] == SDMD_TBDHBFORM || S.evsub[ti] == SDMD_TBDHBBRIK)) {
+        if (((cmap_get(S, ti, nbj) & SDMD_NEIGHBOR_CONNECT) || (cmap_get(S, pj, nbi) & SDMD_NEIGHBOR_CONNECT)) &&
+            S.evsub[ti] == SDMD_TBDHBFORM) {
+            S.hbitype[ti] = SDMD_HBSKIP;
+            return;
+        }
+        double pni[3], pnj[3];
+        image_near(S, V, ti, nbi, pni);
+        image_near(S, V, ti, nbj, pnj);
+        const double a1 = pi0 - pnj[0], a2 = pi1 - pnj[1], a3 = pi2 - pnj[2];
+        const double c1 = pjv[0] - pni[0], c2 = pjv[1] - pni[1], c3 = pjv[2] - pni[2];
+        const double rni2 = a1 * a1 + a2 * a2 + a3 * a3;
+        const double rnj2 = c1 * c1 + c2 * c2 + c3 * c3;
+        double acc[6];
+        acc[0] = hb_energy_change(S, V, ti, pj, 0, err);
+        acc[1] = hb_energy_change(S, V, pj, ti, 0, err);
+        acc[2] = hb_energy_change(S, V, ti, pj, 1, err);
+        acc[3] = hb_energy_change(S, V, pj, ti, 1, err);
+        FP f;
+        find_pair(S, nbj, ti, K_NEIGHBOR, 0, rni2, f, pj, 0, err);
+        acc[4] = f.acc;
+        find_pair(S, nbi, pj, K_NEIGHBOR, 0, rnj2, f, ti, 0, err);
+        acc[5] = f.acc;
+        if (S.evsub[ti] == SDMD_TBDHBFORM) {
+            pot -= hb_barrier(S, ti, pj);
+            pot += (acc[2] - acc[0] + acc[3] - acc[1] + acc[4] + acc[5]);
+        } else {
+            pot += hb_barrier(S, ti, pj);
+            pot += (acc[0] - acc[2] + acc[1] - acc[3] - acc[4] - acc[5]);
+        }
+        if (pot < -0.5 * INFINIT && S.evsub[ti] == SDMD_TBDHBBRIK) { S.hbitype[ti] = SDMD_HBSKIP; return; }
+        if (pot > 0.5 * INFINIT && S.evsub[ti] == SDMD_TBDHBFORM) { S.hbitype[ti] = SDMD_HBSKIP; return; }
+        if (b * b - 2 * d2 * pot / mu > 0) {
+            if (S.evsub[ti] == SDMD_TBDHBFORM) { S.evsub[ti] = SDMD_WELLCAPTR; S.hbitype[ti] = SDMD_HBFORM; }
+            else { S.evsub[ti] = SDMD_WELLESCAP; S.hbitype[ti] = SDMD_HBBREAK; }
+        } else {
+            if (S.evsub[ti] == SDMD_TBDHBFORM) { S.hbitype[ti] = SDMD_HBSKIP; return; }
+            pot = 0;
+            S.evsub[ti] = SDMD_WELLBOUNC;
+            S.hbitype[ti] = SDMD_NOEVENT;
+        }
+    }
+    switch (S.evsub[ti]) {
+    case SDMD_CORECOLLI: s = 1; pot = 0; break;
+    case SDMD_WELLBOUNC: s = (b < 0) ? 1 : -1; pot = 0; break;
+    case SDMD_WELLCAPTR: s = -1; break;
+    case SDMD_WELLESCAP: s = (b < 0) ? -1 : 1; break;
+    case SDMD_TBDCAPTR:
+        if (b * b - 2 * d2 * pot / mu <= 0) { pot = 0; s = 1; } else s = -1;
+        break;
+    case SDMD_TBDESCAP:
+        if (b * b - 2 * d2 * pot / mu <= 0) { pot = 0; s = (b < 0) ? 1 : -1; }
+        else s = (b < 0) ? -1 : 1;
+        break;
+    default: *err = ERR_SUBTYPE; return;
+    }
+    const double phi = (-1 * b + s * sqrt((b * b - 2 * d2 * pot / mu))) / ((S.mass[ti] + S.mass[pj]) * d2);
+    if (isnan(phi)) { *err = ERR_PHI_NAN; return; }
+    const double fi = S.mass[pj] * phi, fj = S.mass[ti] * phi;
+    S.vx[ti] = S.vx[ti] + fi * r1; S.vy[ti] = S.vy[ti] + fi * r2; S.vz[ti] = S.vz[ti] + fi * r3;
+    S.vx[pj] = S.vx[pj] - fj * r1; S.vy[pj] = S.vy[pj] - fj * r2; S.vz[pj] = S.vz[pj] - fj * r3;
+}
+
+
+// HBEvent, Events.c:275-393
+static __device__ void hb_event(Sys &S, int i, int j, int nbi, int nbj, int *err) {
+    int *Pi = S.nbp + (size_t)i * SDMD_NBSLOTS, *Pj = S.nbp + (size_t)j * SDMD_NBSLOTS;
+    int *Pni = S.nbp + (size_t)nbi * SDMD_NBSLOTS, *Pnj = S.nbp + (size_t)nbj * SDMD_NBSLOTS;
+    if (S.hbitype[i] == SDMD_HBFORM) {
+        S.hbbond[i] = j; S.hbbond[j] = i;
+        S.type[i] = type_change(S.type[i], 1);
+        S.type[j] = type_change(S.type[j], 1);
+        if (S.nbstat[i] >= 4 || S.nbstat[j] >= 4 || S.nbstat[nbi] >= 4 || S.nbstat[nbj] >= 4) { *err = ERR_NBOVERFLOW; return; }
+        Pi[++S.nbstat[i]] = S.hbnb[j];
+        Pj[++S.nbstat[j]] = S.hbnb[i];
+        Pni[++S.nbstat[nbi]] = j;
+        Pnj[++S.nbstat[nbj]] = i;
+    } else if (S.hbitype[i] == SDMD_HBBREAK) {
+        S.hbbond[i] = 0; S.hbbond[j] = 0;
+        S.type[i] = type_change(S.type[i], 0);
+        S.type[j] = type_change(S.type[j], 0);
+        --S.nbstat[i]; list_refresh(Pi, S.hbnb[j]);
+        --S.nbstat[j]; list_refresh(Pj, S.hbnb[i]);
+        --S.nbstat[nbi]; list_refresh(Pni, j);
+        --S.nbstat[nbj]; list_refresh(Pnj, i);
+    } else {
+        return;
+    }
+    cmap_toggle(S, i, j, SDMD_HB_CONNECT, err);
+    cmap_toggle(S, j, i, SDMD_HB_CONNECT, err);
+    cmap_toggle(S, i, S.hbnb[j], SDMD_NEIGHBOR_CONNECT, err);
+    cmap_toggle(S, S.hbnb[j], i, SDMD_NEIGHBOR_CONNECT, err);
+    cmap_toggle(S, j, S.hbnb[i], SDMD_NEIGHBOR_CONNECT, err);
+    cmap_toggle(S, S.hbnb[i], j, SDMD_NEIGHBOR_CONNECT, err);
+}
+
+
+}  // namespace sdmd
+
+// ---------------------------------------------------------------- host side
+namespace sdmd { struct FastState; }
+
+struct sdmd_gpu_ctx {
+    int nsys;
+    int block;
+    int mode;
+    hipStream_t stream;
+    hipEvent_t ev0, ev1;
+    sdmd::Sys *host_sys;       // host mirror of the device structs
+    sdmd::Sys *dev_sys;
+    sdmd::FastState *fast;     // screened local-time engine state (SDMD_MODE_FAST)
+    const sdmd_system_desc **desc;
+    void **allocs;
+    int nalloc, capalloc;
+    int *dev_renew;
+    double last_ms;
+};
+
+static int hip_ok(hipError_t e) { return e == hipSuccess ? SDMD_OK : SDMD_EDEVICE; }
+
+template <typename T>
+static int dev_upload(sdmd_gpu_ctx *c, T **dst, const T *src, size_t n) {
+    if (c->nalloc == c->capalloc) {
+        c->capalloc = c->capalloc ? 2 * c->capalloc : 128;
+        c->allocs = (void **)realloc(c->allocs, sizeof(void *) * c->capalloc);
+    }
+    size_t bytes = sizeof(T) * (n ? n : 1);
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return SDMD_EDEVICE;
+    c->allocs[c->nalloc++] = p;
+    if (src && n) {
+        if (hipMemcpy(p, src, sizeof(T) * n, hipMemcpyHostToDevice) != hipSuccess) return SDMD_EDEVICE;
+    } else if (hipMemset(p, 0, bytes) != hipSuccess) {
+        return SDMD_EDEVICE;
+    }
+    *dst = (T *)p;
+    return SDMD_OK;
+}
+
+
+// screened local-time engine (sdmd_fast.hip)
+int sdmd_fast_open(sdmd_gpu_ctx *c, const sdmd_gpu_opts *o);
+int sdmd_fast_prepare(sdmd_gpu_ctx *c);
+int sdmd_fast_run(sdmd_gpu_ctx *c, long long pops, double max_ms);
+int sdmd_fast_download(sdmd_gpu_ctx *c, int s, sdmd_system_desc *out);
+int sdmd_fast_stats(sdmd_gpu_ctx *c, int s, sdmd_gpu_fast_stats *out);
+int sdmd_fast_sync(sdmd_gpu_ctx *c);
+void sdmd_fast_close(sdmd_gpu_ctx *c);

@@ -18,7 +18,9 @@ import numpy as np
 
 from . import _lib
 from ._ctypes_types import DescArrays, SystemDesc
-from ._lib import GpuEvent, GpuOpts, GpuStats, PopRec, check
+from ._lib import FastStats, GpuEvent, GpuOpts, GpuStats, PopRec, check
+
+MODES = {"plain": 0, "fast": 1}
 
 POP_DTYPE = np.dtype([
     ("target", "<i4"), ("partner", "<i4"), ("type", "<i4"), ("sub", "<i4"), ("exec", "<i4"),
@@ -96,17 +98,25 @@ class System:
 
 
 class GpuEngine:
-    """Device-resident systems on one HIP stream (one workgroup per system)."""
+    """Device-resident systems on one HIP stream (one workgroup per system).
+
+    mode "plain": the reference's swept FP64 state, bit-exact event sequence.
+    mode "fast":  local-time state, FP32 screen + FP64 confirm, concurrent
+                  independent events (one per wavefront); `validate` re-solves
+                  every collision candidate in FP64 and counts agreement."""
 
     PHASES = ["pop", "sweep", "event", "stage", "gather", "bondsel", "eval", "select", "hb", "tail", "tree"]
 
     def __init__(self, systems, block=256, trace=False, trace_capacity=0, hash_every=0,
-                 normals_per_pop=5.0, max_pops=0, profile=False):
+                 normals_per_pop=5.0, max_pops=0, profile=False, mode="plain", validate=False,
+                 max_batch=0, seed=0):
         self.systems = list(systems)
+        self.mode = mode
         L = _lib.lib()
         self._descs = [s.desc() for s in self.systems]
         arr = (C.POINTER(SystemDesc) * len(self._descs))(*[C.pointer(d) for d in self._descs])
-        opts = GpuOpts(0, block, 1 if trace else 0, 1 if profile else 0, trace_capacity, hash_every)
+        opts = GpuOpts(MODES[mode], block, 1 if trace else 0, 1 if profile else 0, trace_capacity, hash_every,
+                       1 if validate else 0, max_batch, seed)
         h = C.c_void_p()
         check(L.sdmd_gpu_open(arr, len(self._descs), C.byref(opts), C.byref(h)), "sdmd_gpu_open")
         self._h = h
@@ -115,7 +125,7 @@ class GpuEngine:
         self.hash_every = hash_every
         self._calls = [None] * len(self.systems)
         self._rand0 = [s.arrays.scalars["rand_calls"] for s in self.systems]
-        if max_pops:
+        if max_pops and mode == "plain":
             self.feed_normals(int(normals_per_pop * max_pops) + 4 * max(s.natom for s in self.systems))
 
     def feed_normals(self, n):
@@ -131,6 +141,15 @@ class GpuEngine:
 
     def run(self, pops):
         check(_lib.lib().sdmd_gpu_run(self._h, int(pops)), "sdmd_gpu_run")
+
+    def run_for(self, max_pops, max_ms):
+        """Run until every system did max_pops pops or max_ms of device time passed."""
+        check(_lib.lib().sdmd_gpu_run_for(self._h, int(max_pops), float(max_ms)), "sdmd_gpu_run_for")
+
+    def fast_stats(self, sys=0):
+        st = FastStats()
+        check(_lib.lib().sdmd_gpu_fast_stats_get(self._h, sys, C.byref(st)), "sdmd_gpu_fast_stats_get")
+        return {f: getattr(st, f) for f, _ in FastStats._fields_}
 
     def phase_times(self, sys=0):
         out = np.zeros(len(self.PHASES))
