@@ -188,6 +188,10 @@ void __wrap_CreateCBT(void) {
     }
     __real_CreateCBT();
     if (!g_light) {
+        /* reference energies of the set-up state (CalKinetE / CalPotenE) */
+        FILE *ef = open_out("energy.txt", "w");
+        fprintf(ef, "setup %a %a\n", CalKinetE(), CalPotenE());
+        fclose(ef);
         FILE *f = open_out("tree.bin", "wb");
         fwrite(CBT.node, sizeof(int), 2 * atomnum, f);
         fwrite(CBT.leaf, sizeof(int), atomnum + 1, f);
@@ -251,6 +255,9 @@ void __wrap_UpdateCBT(int *renew) {
             fclose(g_trace);
             fclose(g_hash);
             write_state("final.bin");
+            FILE *ef = open_out("energy.txt", "a");
+            fprintf(ef, "final %a %a\n", CalKinetE(), CalPotenE());
+            fclose(ef);
         }
         exit(0);
     }

@@ -102,7 +102,8 @@ def save_case(name, dump, full, outdir, inp):
                         recal=np.array(int((tr["exec"] == 0).sum())),
                         final_x=r["final"]["x"], final_y=r["final"]["y"], final_z=r["final"]["z"],
                         final_vx=r["final"]["vx"], final_vy=r["final"]["vy"], final_vz=r["final"]["vz"],
-                        final_time=r["final"]["currenttime"])
+                        final_time=r["final"]["currenttime"],
+                        energy_setup=r["energy_setup"], energy_final=r["energy_final"])
     with open(os.path.join(inp, "coordinate.gro")) as fh:
         gro = fh.read()
     with open(os.path.join(inp, "parameter.txt")) as fh:

@@ -96,6 +96,12 @@ def load_dir(d):
         out["hashes"] = read_hashes(os.path.join(d, "hash.bin"))
     if os.path.exists(os.path.join(d, "final.bin")):
         out["final"] = read_state(os.path.join(d, "final.bin"))
+    en = os.path.join(d, "energy.txt")
+    if os.path.exists(en):
+        with open(en) as fh:
+            for ln in fh:
+                tok = ln.split()
+                out["energy_" + tok[0]] = np.array([float.fromhex(tok[1]), float.fromhex(tok[2])])
     tree = os.path.join(d, "tree.bin")
     if os.path.exists(tree):
         raw = np.fromfile(tree, "<i4")
