@@ -131,6 +131,11 @@ int sdmd_gpu_run(sdmd_gpu_ctx *ctx, int64_t pops_per_system);
  * has passed since the launch (max_ms <= 0: no limit).  Aggregate rates over
  * uneven systems are then measured over a common wall-clock window. */
 int sdmd_gpu_run_for(sdmd_gpu_ctx *ctx, int64_t max_pops_per_system, double max_ms);
+/* Run until the system clock passes t_end (the `Time` bound of the loop in
+ * SingleThreadRun, SGThread.c:75: plain mode pops while currenttime <=
+ * t_end; fast mode executes every event with time <= t_end), at most
+ * max_pops pops per system. */
+int sdmd_gpu_run_until(sdmd_gpu_ctx *ctx, double t_end, int64_t max_pops_per_system);
 int sdmd_gpu_fast_stats_get(sdmd_gpu_ctx *ctx, int32_t sys, sdmd_gpu_fast_stats *out);
 
 int sdmd_gpu_predict(sdmd_gpu_ctx *ctx, int32_t sys, const int32_t *renew);

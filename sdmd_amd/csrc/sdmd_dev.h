@@ -31,6 +31,14 @@ enum { ERR_NONE = 0, ERR_TABLE = 1, ERR_HBMODEL = 2, ERR_SUBTYPE = 3, ERR_PHI_NA
        ERR_DYNBITS = 5, ERR_NBLIST = 6, ERR_NBOVERFLOW = 7, ERR_CELL = 8, ERR_LINKLIST = 9,
        ERR_EVENTTYPE = 10, ERR_CELLCAP = 11, ERR_NORMALS = 12, ERR_TRACE = 13 };
 
+// Device arrays are reached through pointers held in LDS-resident structs;
+// casting them to the global address space lets the compiler emit global
+// (not flat) loads and stores, which do not serialise against LDS traffic.
+template <class T>
+using gptr_t = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gptr_t<T> G(T *p) { return (gptr_t<T>)p; }
+
 static __constant__ int c_OX[27] = {0, 1, 0, 0, 0, 1, 1, 1, -1, 0, 0, 0, -1, -1, -1, 0, 0, 1, -1, 1, -1, -1, -1, 1, 1, 1, -1};
 static __constant__ int c_OY[27] = {0, 0, 1, 0, 1, 0, 1, 1, 0, -1, 0, -1, 0, -1, -1, -1, 1, 0, 0, -1, 1, -1, 1, -1, 1, -1, 1};
 static __constant__ int c_OZ[27] = {0, 0, 0, 1, 1, 1, 0, 1, 0, 0, -1, -1, -1, 0, -1, 1, -1, -1, 1, 0, 0, 1, -1, -1, -1, 1, 1};
@@ -112,11 +120,11 @@ static __device__ const sdmd_pair *right_pair(const Sys &S, int t1, int t2) {   
 }
 
 static __device__ int hb_model(const Sys &S, int a1, int a2, int *err) {   // HBModel, Models.c:212-290
-    if (S.role[a1] == 'A' || S.role[a1] == 'S') { int t = a1; a1 = a2; a2 = t; }
-    int o1 = type_change(S.type[a1], 0), o2 = type_change(S.type[a2], 0);
+    if (G(S.role)[a1] == 'A' || G(S.role)[a1] == 'S') { int t = a1; a1 = a2; a2 = t; }
+    int o1 = type_change(G(S.type)[a1], 0), o2 = type_change(G(S.type)[a2], 0);
     if (o1 == 2 && o2 == 26) return 1;
     if (o1 == 2) {
-        switch (S.aatype[a2]) {
+        switch (G(S.aatype)[a2]) {
         case 4: case 7: return 2;
         case 16: case 17: case 19: return 3;
         case 9: return 4;
@@ -129,7 +137,7 @@ static __device__ int hb_model(const Sys &S, int a1, int a2, int *err) {   // HB
     } else if (o1 == 31 && o2 == 31) {
         return 9;
     } else if (o1 == 1) {
-        switch (S.aatype[a2]) {
+        switch (G(S.aatype)[a2]) {
         case 4: case 7: return 6;
         case 16: case 17: case 19: return 7;
         case 9: return 8;
@@ -142,8 +150,8 @@ static __device__ int hb_model(const Sys &S, int a1, int a2, int *err) {   // HB
 }
 
 static __device__ double hb_barrier(const Sys &S, int a1, int a2) {   // HBBarrier, Events.c:1097-1119
-    if (S.role[a1] == 'A' || S.role[a1] == 'S') { int t = a1; a1 = a2; a2 = t; }
-    int o1 = type_change(S.type[a1], 0), o2 = type_change(S.type[a2], 0);
+    if (G(S.role)[a1] == 'A' || G(S.role)[a1] == 'S') { int t = a1; a1 = a2; a2 = t; }
+    int o1 = type_change(G(S.type)[a1], 0), o2 = type_change(G(S.type)[a2], 0);
     if (o1 == 2 && o2 == 26) return S.hbpot[1];
     if (o1 != 2 && o2 != 26) return S.hbpot[5];
     return S.hbpot[3];
@@ -160,19 +168,19 @@ static __device__ int find_pair(const Sys &S, int a1, int a2, int kind, double d
     direction = (direction < 0) ? 1 : -1;
     double tol = direction * ZERO;
     if (kind == K_COLLISION) {
-        int t1 = S.type[a1];
+        int t1 = G(S.type)[a1];
         if (typechange == 1) t1 = type_change(t1, 1);
         else if (typechange == -1) t1 = type_change(t1, 0);
-        c = right_pair(S, t1, S.type[a2]);
+        c = right_pair(S, t1, G(S.type)[a2]);
     } else if (kind == K_CONSTRAINT) {
-        for (int k = S.con_off[a1]; k < S.con_off[a1 + 1]; k++)
-            if (S.con_j[k] == a2) { c = &S.con[k]; break; }
+        for (int k = G(S.con_off)[a1]; k < G(S.con_off)[a1 + 1]; k++)
+            if (G(S.con_j)[k] == a2) { c = &S.con[k]; break; }
     } else if (kind == K_HB) {
         int t = hb_model(S, a1, a2, err);
         c = &S.hbpair[(t * NT + 0) * NT + 0];
     } else {
         int t = hb_model(S, a2, hbpartner, err);
-        int t1 = type_change(S.type[a1], 0), t2 = type_change(S.type[a2], 0);
+        int t1 = type_change(G(S.type)[a1], 0), t2 = type_change(G(S.type)[a2], 0);
         c = &S.hbpair[(t * NT + t1) * NT + t2];
     }
     if (!c || c->dmin == 0) {
@@ -198,36 +206,36 @@ static __device__ int find_pair(const Sys &S, int a1, int a2, int kind, double d
         int ks = 0;
 #pragma unroll
         for (int k = 0; k < 16; k++)
-            if (k < nstep) ks += (d2 >= S.sd[step0 + k] + tol) ? 1 : 0;
+            if (k < nstep) ks += (d2 >= G(S.sd)[step0 + k] + tol) ? 1 : 0;
         if (ks > 0) {
-            f.lo = S.sd[step0 + ks - 1];
-            f.lopot = S.se[step0 + ks - 1];
+            f.lo = G(S.sd)[step0 + ks - 1];
+            f.lopot = G(S.se)[step0 + ks - 1];
         }
         if (ks < nstep) {
             const int k = ks;
-            f.hi = S.sd[step0 + k];
-            f.hipot = -1 * S.se[step0 + k];
-            f.acc = S.sacc[step0 + k];
+            f.hi = G(S.sd)[step0 + k];
+            f.hipot = -1 * G(S.se)[step0 + k];
+            f.acc = G(S.sacc)[step0 + k];
             if (k == 0) return 0;
-            if (kind == K_HB && k == nstep - 1 && S.hbbond[a1] > 0) return 1;
-            if (kind == K_NEIGHBOR && k == nstep - 1 && S.hbbond[a2] > 0) { f.hipot = 0; return 1; }
+            if (kind == K_HB && k == nstep - 1 && G(S.hbbond)[a1] > 0) return 1;
+            if (kind == K_NEIGHBOR && k == nstep - 1 && G(S.hbbond)[a2] > 0) { f.hipot = 0; return 1; }
             return cs;
         }
     } else
     for (int k = 0; k < nstep; k++) {
-        const double sd = S.sd[step0 + k];
+        const double sd = G(S.sd)[step0 + k];
         if (sd == 0) break;
         if (d2 < sd + tol) {
             f.hi = sd;
-            f.hipot = -1 * S.se[step0 + k];
-            f.acc = S.sacc[step0 + k];
+            f.hipot = -1 * G(S.se)[step0 + k];
+            f.acc = G(S.sacc)[step0 + k];
             if (k == 0) return 0;
-            if (kind == K_HB && k == nstep - 1 && S.hbbond[a1] > 0) return 1;
-            if (kind == K_NEIGHBOR && k == nstep - 1 && S.hbbond[a2] > 0) { f.hipot = 0; return 1; }
+            if (kind == K_HB && k == nstep - 1 && G(S.hbbond)[a1] > 0) return 1;
+            if (kind == K_NEIGHBOR && k == nstep - 1 && G(S.hbbond)[a2] > 0) { f.hipot = 0; return 1; }
             return cs;
         }
         f.lo = sd;
-        f.lopot = S.se[step0 + k];
+        f.lopot = G(S.se)[step0 + k];
     }
     if (dmax > 0) {
         if (d2 < dmax + ZERO) {
@@ -253,7 +261,7 @@ static __device__ __forceinline__ void calc_params_at(const Sys &S, double xi, d
                                                       double vyi, double vzi, int cxi, int cyi, int czi, int j,
                                                       double xj, double yj, double zj, double vxj, double vyj,
                                                       double vzj, Kin &k) {
-    const int cxj = S.cx[j], cyj = S.cy[j], czj = S.cz[j];
+    const int cxj = G(S.cx)[j], cyj = G(S.cy)[j], czj = G(S.cz)[j];
     double s1 = 0, s2 = 0, s3 = 0;
     if (cxi != cxj) {
         if (cxi == 0 && cxj == S.cellnum[1] - 1) s1 = -1 * S.box[1];
@@ -281,8 +289,8 @@ static __device__ __forceinline__ void calc_params_at(const Sys &S, double xi, d
 static __device__ __forceinline__ void calc_params(const Sys &S, double xi, double yi, double zi, double vxi,
                                                    double vyi, double vzi, int cxi, int cyi, int czi, int j,
                                                    Kin &k) {
-    calc_params_at(S, xi, yi, zi, vxi, vyi, vzi, cxi, cyi, czi, j, S.x[j], S.y[j], S.z[j], S.vx[j], S.vy[j],
-                   S.vz[j], k);
+    calc_params_at(S, xi, yi, zi, vxi, vyi, vzi, cxi, cyi, czi, j, G(S.x)[j], G(S.y)[j], G(S.z)[j], G(S.vx)[j], G(S.vy)[j],
+                   G(S.vz)[j], k);
 }
 
 static __device__ __forceinline__ double calc_disc(const Kin &k, double shortlimit2) {   // CalculateDisc
@@ -303,16 +311,16 @@ static __device__ __forceinline__ int cell_index(const Sys &S, int cx, int cy, i
 
 // static + dynamic connection-map bits of row i (binary search + slots)
 static __device__ int cmap_get(const Sys &S, int i, int j) {
-    int lo = S.cmap_off[i], hi = S.cmap_off[i + 1] - 1, bits = 0;
+    int lo = G(S.cmap_off)[i], hi = G(S.cmap_off)[i + 1] - 1, bits = 0;
     while (lo <= hi) {
         int m = (lo + hi) >> 1;
-        int v = S.cmap_j[m];
-        if (v == j) { bits = S.cmap_bits[m]; break; }
+        int v = G(S.cmap_j)[m];
+        if (v == j) { bits = G(S.cmap_bits)[m]; break; }
         if (v < j) lo = m + 1; else hi = m - 1;
     }
     const int *dj = S.dynj + (size_t)i * SDMD_DYNSLOTS;
     for (int k = 0; k < SDMD_DYNSLOTS; k++)
-        if (dj[k] == j) { bits |= S.dynb[(size_t)i * SDMD_DYNSLOTS + k]; break; }
+        if (dj[k] == j) { bits |= G(S.dynb)[(size_t)i * SDMD_DYNSLOTS + k]; break; }
     return bits;
 }
 
@@ -342,10 +350,10 @@ static __device__ __forceinline__ void wave_argmin(double &t, int &k) {
 static __device__ int check_pair_match(const Sys &S, int i, int j, int *err) {   // CheckPairMatch 1119-1138
     int t = hb_model(S, i, j, err);
     if (t < 0) return -1;
-    int ti = S.type[i], tj = S.type[j];
-    if (S.hbpair[(t * NT + ti) * NT + tj].nstep == 0) return -1;
-    int ni = type_change(S.type[S.hbnb[i]], 0), nj = type_change(S.type[S.hbnb[j]], 0);
-    if (S.hbpair[(t * NT + ti) * NT + nj].nstep == 0 || S.hbpair[(t * NT + tj) * NT + ni].nstep == 0) return -1;
+    int ti = G(S.type)[i], tj = G(S.type)[j];
+    if (G(S.hbpair)[(t * NT + ti) * NT + tj].nstep == 0) return -1;
+    int ni = type_change(G(S.type)[G(S.hbnb)[i]], 0), nj = type_change(G(S.type)[G(S.hbnb)[j]], 0);
+    if (G(S.hbpair)[(t * NT + ti) * NT + nj].nstep == 0 || G(S.hbpair)[(t * NT + tj) * NT + ni].nstep == 0) return -1;
     return 0;
 }
 
@@ -368,14 +376,14 @@ static __device__ void list_refresh(int *l, int target) {   // ListRefresh(targe
 // walk the atoms of a cell (linked list or cell array).
 struct PosStored {
     const Sys &S;
-    __device__ void pos(int j, double *p) const { p[0] = S.x[j]; p[1] = S.y[j]; p[2] = S.z[j]; }
-    __device__ int first(int cell) const { return S.celllist[S.N + cell]; }
-    __device__ int next(int, int j) const { return S.celllist[j]; }
+    __device__ void pos(int j, double *p) const { p[0] = G(S.x)[j]; p[1] = G(S.y)[j]; p[2] = G(S.z)[j]; }
+    __device__ int first(int cell) const { return G(S.celllist)[S.N + cell]; }
+    __device__ int next(int, int j) const { return G(S.celllist)[j]; }
 };
 
 template <class P>
 static __device__ void image_near(const Sys &S, const P &V, int i, int j, double *p) {
-    const int ci[3] = {S.cx[i], S.cy[i], S.cz[i]}, cj[3] = {S.cx[j], S.cy[j], S.cz[j]};
+    const int ci[3] = {G(S.cx)[i], G(S.cy)[i], G(S.cz)[i]}, cj[3] = {G(S.cx)[j], G(S.cy)[j], G(S.cz)[j]};
     V.pos(j, p);
     for (int n = 0; n < 3; n++) {
         if (ci[n] != cj[n]) {
@@ -392,7 +400,7 @@ static __device__ double hb_energy_change(const Sys &S, const P &V, int t, int s
     double energy = 0, acc = 0;
     for (int c = 0; c < 27; c++) {
         int guard = 0;
-        const int cell = cell_index(S, S.cx[t], S.cy[t], S.cz[t], c);
+        const int cell = cell_index(S, G(S.cx)[t], G(S.cy)[t], G(S.cz)[t], c);
         for (int j = V.first(cell); j; j = V.next(cell, j)) {
             if (++guard > S.N) { *err = ERR_LINKLIST; return energy; }
             const int bits = cmap_get(S, t, j);
@@ -401,14 +409,14 @@ static __device__ double hb_energy_change(const Sys &S, const P &V, int t, int s
                 double pt[3], pj[3];
                 V.pos(t, pt);
                 V.pos(j, pj);
-                calc_params_at(S, pt[0], pt[1], pt[2], 0, 0, 0, S.cx[t], S.cy[t], S.cz[t], j, pj[0], pj[1], pj[2],
-                               S.vx[j], S.vy[j], S.vz[j], k);
+                calc_params_at(S, pt[0], pt[1], pt[2], 0, 0, 0, G(S.cx)[t], G(S.cy)[t], G(S.cz)[t], j, pj[0], pj[1], pj[2],
+                               G(S.vx)[j], G(S.vy)[j], G(S.vz)[j], k);
                 if (k.rr <= cut) {
                     FP f;
                     if (hbtype == 0) {
                         find_pair(S, t, j, K_COLLISION, 0, k.rr, f, 0, -1, err);
                         acc = f.acc;
-                    } else if (j != S.hbnb[skip]) {
+                    } else if (j != G(S.hbnb)[skip]) {
                         find_pair(S, t, j, K_COLLISION, 0, k.rr, f, 0, 1, err);
                         acc = f.acc;
                     }
@@ -430,16 +438,16 @@ static __device__ void interaction_event(Sys &S, const P &V, int ti, int pj, int
     double pjv[3];
     image_near(S, V, ti, pj, pjv);
     const double r1 = pi0 - pjv[0], r2 = pi1 - pjv[1], r3 = pi2 - pjv[2];
-    const double v1 = S.vx[ti] - S.vx[pj], v2 = S.vy[ti] - S.vy[pj], v3 = S.vz[ti] - S.vz[pj];
+    const double v1 = G(S.vx)[ti] - G(S.vx)[pj], v2 = G(S.vy)[ti] - G(S.vy)[pj], v3 = G(S.vz)[ti] - G(S.vz)[pj];
     const double d2 = r1 * r1 + r2 * r2 + r3 * r3;
     const double b = r1 * v1 + r2 * v2 + r3 * v3;
-    const double mu = S.mass[ti] * S.mass[pj] / (S.mass[ti] + S.mass[pj]);
-    double pot = S.evpot[ti];
+    const double mu = G(S.mass)[ti] * G(S.mass)[pj] / (G(S.mass)[ti] + G(S.mass)[pj]);
+    double pot = G(S.evpot)[ti];
     int s = 0;
-    if (S.evtype[ti] == SDMD_HB && (S.evsub[ti] == SDMD_TBDHBFORM || S.evsub[ti] == SDMD_TBDHBBRIK)) {
+    if (G(S.evtype)[ti] == SDMD_HB && (G(S.evsub)[ti] == SDMD_TBDHBFORM || G(S.evsub)[ti] == SDMD_TBDHBBRIK)) {
         if (((cmap_get(S, ti, nbj) & SDMD_NEIGHBOR_CONNECT) || (cmap_get(S, pj, nbi) & SDMD_NEIGHBOR_CONNECT)) &&
-            S.evsub[ti] == SDMD_TBDHBFORM) {
-            S.hbitype[ti] = SDMD_HBSKIP;
+            G(S.evsub)[ti] == SDMD_TBDHBFORM) {
+            G(S.hbitype)[ti] = SDMD_HBSKIP;
             return;
         }
         double pni[3], pnj[3];
@@ -459,26 +467,26 @@ static __device__ void interaction_event(Sys &S, const P &V, int ti, int pj, int
         acc[4] = f.acc;
         find_pair(S, nbi, pj, K_NEIGHBOR, 0, rnj2, f, ti, 0, err);
         acc[5] = f.acc;
-        if (S.evsub[ti] == SDMD_TBDHBFORM) {
+        if (G(S.evsub)[ti] == SDMD_TBDHBFORM) {
             pot -= hb_barrier(S, ti, pj);
             pot += (acc[2] - acc[0] + acc[3] - acc[1] + acc[4] + acc[5]);
         } else {
             pot += hb_barrier(S, ti, pj);
             pot += (acc[0] - acc[2] + acc[1] - acc[3] - acc[4] - acc[5]);
         }
-        if (pot < -0.5 * INFINIT && S.evsub[ti] == SDMD_TBDHBBRIK) { S.hbitype[ti] = SDMD_HBSKIP; return; }
-        if (pot > 0.5 * INFINIT && S.evsub[ti] == SDMD_TBDHBFORM) { S.hbitype[ti] = SDMD_HBSKIP; return; }
+        if (pot < -0.5 * INFINIT && G(S.evsub)[ti] == SDMD_TBDHBBRIK) { G(S.hbitype)[ti] = SDMD_HBSKIP; return; }
+        if (pot > 0.5 * INFINIT && G(S.evsub)[ti] == SDMD_TBDHBFORM) { G(S.hbitype)[ti] = SDMD_HBSKIP; return; }
         if (b * b - 2 * d2 * pot / mu > 0) {
-            if (S.evsub[ti] == SDMD_TBDHBFORM) { S.evsub[ti] = SDMD_WELLCAPTR; S.hbitype[ti] = SDMD_HBFORM; }
-            else { S.evsub[ti] = SDMD_WELLESCAP; S.hbitype[ti] = SDMD_HBBREAK; }
+            if (G(S.evsub)[ti] == SDMD_TBDHBFORM) { G(S.evsub)[ti] = SDMD_WELLCAPTR; G(S.hbitype)[ti] = SDMD_HBFORM; }
+            else { G(S.evsub)[ti] = SDMD_WELLESCAP; G(S.hbitype)[ti] = SDMD_HBBREAK; }
         } else {
-            if (S.evsub[ti] == SDMD_TBDHBFORM) { S.hbitype[ti] = SDMD_HBSKIP; return; }
+            if (G(S.evsub)[ti] == SDMD_TBDHBFORM) { G(S.hbitype)[ti] = SDMD_HBSKIP; return; }
             pot = 0;
-            S.evsub[ti] = SDMD_WELLBOUNC;
-            S.hbitype[ti] = SDMD_NOEVENT;
+            G(S.evsub)[ti] = SDMD_WELLBOUNC;
+            G(S.hbitype)[ti] = SDMD_NOEVENT;
         }
     }
-    switch (S.evsub[ti]) {
+    switch (G(S.evsub)[ti]) {
     case SDMD_CORECOLLI: s = 1; pot = 0; break;
     case SDMD_WELLBOUNC: s = (b < 0) ? 1 : -1; pot = 0; break;
     case SDMD_WELLCAPTR: s = -1; break;
@@ -492,11 +500,11 @@ static __device__ void interaction_event(Sys &S, const P &V, int ti, int pj, int
         break;
     default: *err = ERR_SUBTYPE; return;
     }
-    const double phi = (-1 * b + s * sqrt((b * b - 2 * d2 * pot / mu))) / ((S.mass[ti] + S.mass[pj]) * d2);
+    const double phi = (-1 * b + s * sqrt((b * b - 2 * d2 * pot / mu))) / ((G(S.mass)[ti] + G(S.mass)[pj]) * d2);
     if (isnan(phi)) { *err = ERR_PHI_NAN; return; }
-    const double fi = S.mass[pj] * phi, fj = S.mass[ti] * phi;
-    S.vx[ti] = S.vx[ti] + fi * r1; S.vy[ti] = S.vy[ti] + fi * r2; S.vz[ti] = S.vz[ti] + fi * r3;
-    S.vx[pj] = S.vx[pj] - fj * r1; S.vy[pj] = S.vy[pj] - fj * r2; S.vz[pj] = S.vz[pj] - fj * r3;
+    const double fi = G(S.mass)[pj] * phi, fj = G(S.mass)[ti] * phi;
+    G(S.vx)[ti] = G(S.vx)[ti] + fi * r1; G(S.vy)[ti] = G(S.vy)[ti] + fi * r2; G(S.vz)[ti] = G(S.vz)[ti] + fi * r3;
+    G(S.vx)[pj] = G(S.vx)[pj] - fj * r1; G(S.vy)[pj] = G(S.vy)[pj] - fj * r2; G(S.vz)[pj] = G(S.vz)[pj] - fj * r3;
 }
 
 
@@ -504,32 +512,32 @@ static __device__ void interaction_event(Sys &S, const P &V, int ti, int pj, int
 static __device__ void hb_event(Sys &S, int i, int j, int nbi, int nbj, int *err) {
     int *Pi = S.nbp + (size_t)i * SDMD_NBSLOTS, *Pj = S.nbp + (size_t)j * SDMD_NBSLOTS;
     int *Pni = S.nbp + (size_t)nbi * SDMD_NBSLOTS, *Pnj = S.nbp + (size_t)nbj * SDMD_NBSLOTS;
-    if (S.hbitype[i] == SDMD_HBFORM) {
-        S.hbbond[i] = j; S.hbbond[j] = i;
-        S.type[i] = type_change(S.type[i], 1);
-        S.type[j] = type_change(S.type[j], 1);
-        if (S.nbstat[i] >= 4 || S.nbstat[j] >= 4 || S.nbstat[nbi] >= 4 || S.nbstat[nbj] >= 4) { *err = ERR_NBOVERFLOW; return; }
-        Pi[++S.nbstat[i]] = S.hbnb[j];
-        Pj[++S.nbstat[j]] = S.hbnb[i];
-        Pni[++S.nbstat[nbi]] = j;
-        Pnj[++S.nbstat[nbj]] = i;
-    } else if (S.hbitype[i] == SDMD_HBBREAK) {
-        S.hbbond[i] = 0; S.hbbond[j] = 0;
-        S.type[i] = type_change(S.type[i], 0);
-        S.type[j] = type_change(S.type[j], 0);
-        --S.nbstat[i]; list_refresh(Pi, S.hbnb[j]);
-        --S.nbstat[j]; list_refresh(Pj, S.hbnb[i]);
-        --S.nbstat[nbi]; list_refresh(Pni, j);
-        --S.nbstat[nbj]; list_refresh(Pnj, i);
+    if (G(S.hbitype)[i] == SDMD_HBFORM) {
+        G(S.hbbond)[i] = j; G(S.hbbond)[j] = i;
+        G(S.type)[i] = type_change(G(S.type)[i], 1);
+        G(S.type)[j] = type_change(G(S.type)[j], 1);
+        if (G(S.nbstat)[i] >= 4 || G(S.nbstat)[j] >= 4 || G(S.nbstat)[nbi] >= 4 || G(S.nbstat)[nbj] >= 4) { *err = ERR_NBOVERFLOW; return; }
+        Pi[++G(S.nbstat)[i]] = G(S.hbnb)[j];
+        Pj[++G(S.nbstat)[j]] = G(S.hbnb)[i];
+        Pni[++G(S.nbstat)[nbi]] = j;
+        Pnj[++G(S.nbstat)[nbj]] = i;
+    } else if (G(S.hbitype)[i] == SDMD_HBBREAK) {
+        G(S.hbbond)[i] = 0; G(S.hbbond)[j] = 0;
+        G(S.type)[i] = type_change(G(S.type)[i], 0);
+        G(S.type)[j] = type_change(G(S.type)[j], 0);
+        --G(S.nbstat)[i]; list_refresh(Pi, G(S.hbnb)[j]);
+        --G(S.nbstat)[j]; list_refresh(Pj, G(S.hbnb)[i]);
+        --G(S.nbstat)[nbi]; list_refresh(Pni, j);
+        --G(S.nbstat)[nbj]; list_refresh(Pnj, i);
     } else {
         return;
     }
     cmap_toggle(S, i, j, SDMD_HB_CONNECT, err);
     cmap_toggle(S, j, i, SDMD_HB_CONNECT, err);
-    cmap_toggle(S, i, S.hbnb[j], SDMD_NEIGHBOR_CONNECT, err);
-    cmap_toggle(S, S.hbnb[j], i, SDMD_NEIGHBOR_CONNECT, err);
-    cmap_toggle(S, j, S.hbnb[i], SDMD_NEIGHBOR_CONNECT, err);
-    cmap_toggle(S, S.hbnb[i], j, SDMD_NEIGHBOR_CONNECT, err);
+    cmap_toggle(S, i, G(S.hbnb)[j], SDMD_NEIGHBOR_CONNECT, err);
+    cmap_toggle(S, G(S.hbnb)[j], i, SDMD_NEIGHBOR_CONNECT, err);
+    cmap_toggle(S, j, G(S.hbnb)[i], SDMD_NEIGHBOR_CONNECT, err);
+    cmap_toggle(S, G(S.hbnb)[i], j, SDMD_NEIGHBOR_CONNECT, err);
 }
 
 
@@ -579,7 +587,7 @@ static int dev_upload(sdmd_gpu_ctx *c, T **dst, const T *src, size_t n) {
 // screened local-time engine (sdmd_fast.hip)
 int sdmd_fast_open(sdmd_gpu_ctx *c, const sdmd_gpu_opts *o);
 int sdmd_fast_prepare(sdmd_gpu_ctx *c);
-int sdmd_fast_run(sdmd_gpu_ctx *c, long long pops, double max_ms);
+int sdmd_fast_run(sdmd_gpu_ctx *c, long long pops, double max_ms, double t_end);
 int sdmd_fast_download(sdmd_gpu_ctx *c, int s, sdmd_system_desc *out);
 int sdmd_fast_stats(sdmd_gpu_ctx *c, int s, sdmd_gpu_fast_stats *out);
 int sdmd_fast_sync(sdmd_gpu_ctx *c);

@@ -43,26 +43,35 @@
 
 namespace sdmd {
 
+typedef float fvec4 __attribute__((ext_vector_type(4)));   // mirror record half (16 B)
+
 constexpr int FCAP = 64;          // atoms per cell
 constexpr int FMAXBLK = 1024;     // 64-atom blocks per system (N <= 65535)
 constexpr int FSEL = 64;          // events considered per round
 constexpr int FROW = 96;          // connection-map row staged per wave
 constexpr int FMAXW = 16;         // wavefronts per workgroup
+constexpr long long FDRAIN = 1 << 20;   // bound on the pops of a closing drain
 constexpr float FINF = 3.0e38f;
 
 enum { FERR_SELECT = 20, FERR_SLOT = 21, FERR_ROW = 22, FERR_CAND = 23, FERR_NBLK = 24 };
+
+// phases timed (by wavefront 0) when profiling is on
+enum { FPH_SELECT, FPH_EXEC, FPH_STAGE, FPH_SCREEN, FPH_CONFIRM, FPH_OTHER, FPH_TREE, FPH_EVENT, FPH_BOND,
+       FPH_SCAN, FPH_COLLECT };
 
 struct FSys {
     Sys s;
     double *t0;                 // local time of (x, v)
     int *ccnt, *cmem, *slot;    // cell arrays: ccnt[cell], cmem[cell*FCAP+q], slot[atom]
-    float4 *rec;                // FP32 mirror, two float4 per slot
-    double *b1, *b2;            // block minimum / second minimum (persisted)
-    int *ba;                    // atom of the block minimum
+    fvec4 *rec;                // FP32 mirror, two fvec4 per slot
+    double *b1, *b2, *b3;       // block's earliest, second and third event times (persisted)
+    int *ba, *bb;               // atoms of the earliest and second event
     unsigned int *rctr;         // per-atom draw counter
     int ncell, nblk;
     float L32[3];
     double Te, rebase_dt;
+    double t_stop;              // events later than this stay pending (run_until)
+    const void *scrtab;         // ScrTab (FP32 collision table), copied into LDS per launch
     unsigned int key0, key1;
     int validate, max_batch;
     // statistics
@@ -85,8 +94,8 @@ __device__ __forceinline__ void philox10(unsigned int c[4], unsigned int k0, uns
 }
 
 __device__ double fast_normal(FSys &F, int a) {
-    unsigned int c[4] = {(unsigned int)a, F.rctr[a], 0x5344u, 0x4d44u};
-    F.rctr[a] = c[1] + 1;
+    unsigned int c[4] = {(unsigned int)a, G(F.rctr)[a], 0x5344u, 0x4d44u};
+    G(F.rctr)[a] = c[1] + 1;
     philox10(c, F.key0, F.key1);
     const unsigned long long w1 = ((unsigned long long)c[0] << 32) | c[1];
     const unsigned long long w2 = ((unsigned long long)c[2] << 32) | c[3];
@@ -111,6 +120,18 @@ __device__ __forceinline__ void cnt_max(double &d, double v) {
     }
 }
 
+// ---------------------------------------------------------------- uniform values
+// Values every lane of a wave holds alike (the predicted atom's fields) are
+// moved to scalar registers, which frees vector registers for the candidates.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ long long uni(long long v) {
+    const int lo = __builtin_amdgcn_readfirstlane((int)v);
+    const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double uni(double v) { return __longlong_as_double(uni(__double_as_longlong(v))); }
+__device__ __forceinline__ float uni(float v) { return __int_as_float(uni(__float_as_int(v))); }
+
 // ---------------------------------------------------------------- state access
 __device__ __forceinline__ int cell_id(const Sys &S, int cx, int cy, int cz) {
     return cz * S.cellnum[1] * S.cellnum[2] + cy * S.cellnum[1] + cx + 1;
@@ -121,53 +142,60 @@ struct PosLocal {      // positions at time t from the local-time state, cells f
     double t;
     __device__ void pos(int j, double *p) const {
         const Sys &S = F.s;
-        const double dt = t - F.t0[j];
-        p[0] = S.x[j] + S.vx[j] * dt;
-        p[1] = S.y[j] + S.vy[j] * dt;
-        p[2] = S.z[j] + S.vz[j] * dt;
+        const double dt = t - G(F.t0)[j];
+        p[0] = G(S.x)[j] + G(S.vx)[j] * dt;
+        p[1] = G(S.y)[j] + G(S.vy)[j] * dt;
+        p[2] = G(S.z)[j] + G(S.vz)[j] * dt;
     }
-    __device__ int first(int cell) const { return F.ccnt[cell] > 0 ? F.cmem[(size_t)cell * 64] : 0; }
+    __device__ int first(int cell) const { return G(F.ccnt)[cell] > 0 ? G(F.cmem)[(size_t)cell * 64] : 0; }
     __device__ int next(int cell, int j) const {
-        const int q = F.slot[j] + 1;
-        return (q > 0 && q < F.ccnt[cell] && q < 64) ? F.cmem[(size_t)cell * 64 + q] : 0;
+        const int q = G(F.slot)[j] + 1;
+        return (q > 0 && q < G(F.ccnt)[cell] && q < 64) ? G(F.cmem)[(size_t)cell * 64 + q] : 0;
     }
 };
 
 __device__ __forceinline__ void advance(FSys &F, int a, double t) {
     Sys &S = F.s;
-    const double dt = t - F.t0[a];
-    S.x[a] = S.x[a] + S.vx[a] * dt;
-    S.y[a] = S.y[a] + S.vy[a] * dt;
-    S.z[a] = S.z[a] + S.vz[a] * dt;
-    F.t0[a] = t;
+    const double dt = t - G(F.t0)[a];
+    G(S.x)[a] = G(S.x)[a] + G(S.vx)[a] * dt;
+    G(S.y)[a] = G(S.y)[a] + G(S.vy)[a] * dt;
+    G(S.z)[a] = G(S.z)[a] + G(S.vz)[a] * dt;
+    G(F.t0)[a] = t;
 }
 
 // mirror record of atom a (position at t0, expressed at epoch Te)
 __device__ void put_record(FSys &F, int a) {
     const Sys &S = F.s;
-    const int cx = S.cx[a], cy = S.cy[a], cz = S.cz[a];
-    const size_t at = (size_t)cell_id(S, cx, cy, cz) * FCAP + F.slot[a];
-    float4 r0, r1;
-    r0.x = (float)(S.x[a] - cx * S.cellsize[1]);
-    r0.y = (float)(S.y[a] - cy * S.cellsize[2]);
-    r0.z = (float)(S.z[a] - cz * S.cellsize[3]);
-    r0.w = (float)(F.t0[a] - F.Te);
-    r1.x = (float)S.vx[a];
-    r1.y = (float)S.vy[a];
-    r1.z = (float)S.vz[a];
-    r1.w = __int_as_float((a << 8) | (S.type[a] & 255));
-    F.rec[2 * at] = r0;
-    F.rec[2 * at + 1] = r1;
+    const int cx = G(S.cx)[a], cy = G(S.cy)[a], cz = G(S.cz)[a];
+    const size_t at = (size_t)cell_id(S, cx, cy, cz) * FCAP + G(F.slot)[a];
+    fvec4 r0, r1;
+    r0.x = (float)(G(S.x)[a] - cx * S.cellsize[1]);
+    r0.y = (float)(G(S.y)[a] - cy * S.cellsize[2]);
+    r0.z = (float)(G(S.z)[a] - cz * S.cellsize[3]);
+    r0.w = (float)(G(F.t0)[a] - F.Te);
+    r1.x = (float)G(S.vx)[a];
+    r1.y = (float)G(S.vy)[a];
+    r1.z = (float)G(S.vz)[a];
+    r1.w = __int_as_float((a << 8) | (G(S.type)[a] & 255));
+    G(F.rec)[2 * at] = r0;
+    G(F.rec)[2 * at + 1] = r1;
 }
 
 // ---------------------------------------------------------------- predicted atom
 struct FAtom {
     double x, y, z, vx, vy, vz, t;
     long long counter;
-    int i, cx, cy, cz, type, hbbond, hbnb, protein, aanum, nbstat, nbnum, nrow;
+    int i, cx, cy, cz, type, hbbond, hbnb, protein, aanum, nbstat, nbnum, nrow, rowlo, rowhi;
+    int spanok;                  // the static row spans < 128 atom ids: bitmask lookup
+    int ncon, constaged;         // constraint partners; all staged in LDS
+    unsigned long long mb[2], mc[2];   // BOND / CONSTRAINT bits over rowlo .. rowlo+127
     signed char role;
     int nbp[SDMD_NBSLOTS];
+    fvec4 s0, s1;            // the atom's own mirror record (position at t, velocity)
 };
+
+// statistics accumulated per wave, added to the system at the end
+struct WStats { long long screen, conf, fp64, cand; };
 
 struct Best {
     double t, pot;
@@ -179,22 +207,30 @@ __device__ __forceinline__ void take(Best &B, const Sys &S, double t, double pot
     // JobAssign, TimePrediction.c:1140-1158: strictly earlier and positive
     if (t > 0 && B.t > t) {
         B.t = t; B.pot = pot; B.sub = sub; B.type = type;
-        if (j) { B.partner = j; B.pc = S.evc[j]; }
+        if (j) { B.partner = j; B.pc = G(S.evc)[j]; }
         else { B.partner = INVALID; B.pc = 0; }
     }
 }
+
+constexpr int FCON = 16;          // constraint partners staged per predicted atom
+constexpr int FCONSTEP = 8;       // steps per staged constraint
 
 struct FWave {
     int row_j[FROW], row_b[FROW];
     int dyn_j[SDMD_DYNSLOTS], dyn_b[SDMD_DYNSLOTS];
     int coff[28];
     int cell[27];
+    // the predicted atom's constraint shells (FP32), CONSTRn lists
+    int con_j[FCON], con_nstep[FCON];
+    unsigned int con_ez[FCON];
+    float con_dmin[FCON], con_dmax[FCON];
+    float con_rad[FCON][FCONSTEP];
 };
 
 __device__ __forceinline__ int frow_bits(const FWave &wm, const Sys &S, const FAtom &A, int j) {
     int bits = 0;
     if (A.nrow <= FROW) {
-        int lo = 0, hi = A.nrow - 1;
+        int lo = 0, hi = (j < A.rowlo || j > A.rowhi) ? -1 : A.nrow - 1;   // outside the row's span: absent
         while (lo <= hi) {
             const int m = (lo + hi) >> 1;
             const int v = wm.row_j[m];
@@ -206,6 +242,19 @@ __device__ __forceinline__ int frow_bits(const FWave &wm, const Sys &S, const FA
         return bits;
     }
     return cmap_get(S, A.i, j);
+}
+
+// BOND / CONSTRAINT bits of (A, j), the only connection bits CollisionTime
+// tests (TimePrediction.c:96-118); the dynamic slots carry H-bond bits only
+__device__ __forceinline__ int coli_bits(const FWave &wm, const Sys &S, const FAtom &A, int j) {
+    if (A.spanok) {
+        const unsigned off = (unsigned)(j - A.rowlo);
+        if (off >= 128u) return 0;
+        const int w = off >> 6, sh = off & 63;
+        const unsigned long long mb = w ? A.mb[1] : A.mb[0], mc = w ? A.mc[1] : A.mc[0];
+        return (int)((mb >> sh) & 1) * SDMD_BOND_CONNECT | (int)((mc >> sh) & 1) * SDMD_CONSTRAINT_CONNECT;
+    }
+    return frow_bits(wm, S, A, j) & (SDMD_BOND_CONNECT | SDMD_CONSTRAINT_CONNECT);
 }
 
 __device__ __forceinline__ int fin_nb_list(const FAtom &A, int j) {
@@ -223,10 +272,10 @@ __device__ __forceinline__ int fin_nb_list(const FAtom &A, int j) {
 // pair kinematics for atom A (at its time) and j (local state advanced to A.t)
 __device__ __forceinline__ void kin_local(const FSys &F, const FAtom &A, int j, Kin &k) {
     const Sys &S = F.s;
-    const double dt = A.t - F.t0[j];
-    const double vxj = S.vx[j], vyj = S.vy[j], vzj = S.vz[j];
-    calc_params_at(S, A.x, A.y, A.z, A.vx, A.vy, A.vz, A.cx, A.cy, A.cz, j, S.x[j] + vxj * dt, S.y[j] + vyj * dt,
-                   S.z[j] + vzj * dt, vxj, vyj, vzj, k);
+    const double dt = A.t - G(F.t0)[j];
+    const double vxj = G(S.vx)[j], vyj = G(S.vy)[j], vzj = G(S.vz)[j];
+    calc_params_at(S, A.x, A.y, A.z, A.vx, A.vy, A.vz, A.cx, A.cy, A.cz, j, G(S.x)[j] + vxj * dt, G(S.y)[j] + vyj * dt,
+                   G(S.z)[j] + vzj * dt, vxj, vyj, vzj, k);
 }
 
 // one CollisionTime candidate in FP64 (TimePrediction.c:79-179), exclusions
@@ -269,39 +318,56 @@ __device__ double coli64(const FSys &F, const FAtom &A, int j, int bits, double 
 // or flag when a decision lies inside the error band (FP64 decides then)
 struct Scr { float t, e; bool flag; };
 
-__device__ Scr screen32(const Sys &S, const sdmd_pair *c, bool con, float rr, float b, float v2) {
+// One shell description for the screen: hard core, step radii, outer
+// limit (all squared), and bit k set when step k carries no energy.
+struct Shell {
+    float dmin, dmax;
+    int nstep;
+    unsigned int ezero;
+};
+
+// FP32 shell lookup + contact time; `rad(k)` gives step radius k
+template <class R>
+__device__ __forceinline__ Scr screen_shell(const Shell &c, const R &rad, bool con, float rr, float b, float v2) {
     Scr o;
     o.t = FINF; o.e = 0; o.flag = false;
-    if (!c) { o.flag = true; return o; }
     const float band = 2e-5f * rr + 1e-5f;
-    const float dmin = (float)c->dmin, dmax = (float)c->dmax;
-    const int step0 = c->step0, nstep = c->nstep;
-    if (rr < dmin + band) { o.flag = true; return o; }     // core boundary or inside the core
+    if (rr < c.dmin + band) { o.flag = true; return o; }   // core boundary or inside the core
+    // count the radii passed (no early exit: the loads issue together)
     int ks = 0;
-    for (int k = 0; k < nstep; k++) {
-        const float d = (float)S.sd[step0 + k];
-        if (fabsf(rr - d) < band) { o.flag = true; return o; }
-        ks += (rr > d) ? 1 : 0;
+    bool near = false;
+    float lo = c.dmin, hi = FINF;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        if (k < c.nstep) {
+            const float d = rad(k);
+            near |= fabsf(rr - d) < band;
+            if (rr > d) { ks++; lo = d; }
+            else if (hi == FINF) hi = d;
+        }
     }
-    float lo, hi;
-    bool lopot0, hipot0;
+    if (near) { o.flag = true; return o; }
+    const bool lopot0 = ks ? ((c.ezero >> (ks - 1)) & 1) : true;
+    bool hipot0;
     int cs;
-    lo = ks ? (float)S.sd[step0 + ks - 1] : dmin;
-    lopot0 = ks ? (S.se[step0 + ks - 1] == 0) : true;
     const float eb = 1e-5f * sqrtf(rr * v2) + 4e-6f * sqrtf(v2) + 1e-20f;
     if (fabsf(b) <= eb) { o.flag = true; return o; }
-    if (ks < nstep) {
-        hi = (float)S.sd[step0 + ks];
-        hipot0 = (S.se[step0 + ks] == 0);
+    if (ks < c.nstep) {
+        hipot0 = (c.ezero >> ks) & 1;
         cs = ks == 0 ? 0 : -1;
-    } else if (dmax > 0) {
-        if (rr > dmax - band) { o.flag = true; return o; }  // at or beyond the outer limit
-        hi = dmax;
+    } else if (c.dmax > 0) {
+        if (rr > c.dmax - band) { o.flag = true; return o; }   // at or beyond the outer limit
+        hi = c.dmax;
         hipot0 = true;
-        cs = (nstep == 0 && b < 0) ? 0 : 1;
+        cs = (c.nstep == 0 && b < 0) ? 0 : 1;
     } else {
-        o.flag = con;                                        // unbounded shell: no event for a collision
-        return o;
+        // beyond the last step with no outer limit: an approaching pair still
+        // reaches the outermost step; a leaving one has no event (collision)
+        // or an unbounded one (constraint: left to FP64)
+        if (con && !(b < 0)) { o.flag = true; return o; }
+        hi = FINF;
+        hipot0 = false;
+        cs = 1;
     }
     if (!con && lopot0 && hipot0) return o;
     const float disc = b * b - v2 * (rr - lo);
@@ -328,6 +394,20 @@ __device__ Scr screen32(const Sys &S, const sdmd_pair *c, bool con, float rr, fl
     if (!(o.t > o.e)) o.flag = true;                         // sign of t uncertain
     return o;
 }
+
+// compact FP32 copy of the collision table (RightPair resolved), held in LDS
+constexpr int FRAD = 8192;
+struct ScrPair {
+    float dmin, dmax;
+    unsigned short off;          // first radius in the FP32 pool
+    unsigned short nstep;
+    unsigned int ezero;
+};
+struct ScrTab {
+    short rp[NT * NT];           // resolved pair per (type_i, type_j), -1 = none
+    ScrPair pr[NT * NT];
+    float rad[FRAD];
+};
 
 // ---------------------------------------------------------------- wave helpers
 __device__ __forceinline__ void wave_argmin_f(float &t, int &k) {
@@ -381,36 +461,120 @@ __device__ __forceinline__ void wave_best(Best &mine, int k, const Best &running
 }
 
 // ---------------------------------------------------------------- Predict (one atom, one wave)
-__device__ void fstage(FSys &F, FWave &wm, FAtom &A, int i, double t) {
+// Stage the predicted atom: its fields, its connection row and dynamic
+// slots (LDS), the 27 neighbour cells and their occupancy prefix (LDS), and
+// the bond entry of this lane (lanes < bond count).  No memory fence: the
+// LDS data is shared within the wave only, whose LDS operations execute in
+// order, so independent global loads stay in flight across the stages.
+struct BondPre {
+    int j, sub;
+    double lo, hi, xj, yj, zj, vxj, vyj, vzj, t0j;
+    long long cj;
+};
+
+__device__ __forceinline__ void fstage(FSys &F, FWave &wm, FAtom &A, int i, double t, BondPre &bp, int &nb) {
     const Sys &S = F.s;
     const int lane = threadIdx.x & 63;
+    i = uni(i);
+    t = uni(t);
     A.i = i; A.t = t;
-    A.x = S.x[i]; A.y = S.y[i]; A.z = S.z[i];
-    A.vx = S.vx[i]; A.vy = S.vy[i]; A.vz = S.vz[i];
-    A.counter = S.evc[i];
-    A.cx = S.cx[i]; A.cy = S.cy[i]; A.cz = S.cz[i];
-    A.type = S.type[i]; A.hbbond = S.hbbond[i]; A.hbnb = S.hbnb[i];
-    A.protein = S.protein[i]; A.aanum = S.aanum[i]; A.role = S.role[i];
-    A.nbstat = S.nbstat[i]; A.nbnum = S.nbnum[i];
-    for (int k = 0; k < SDMD_NBSLOTS; k++) A.nbp[k] = S.nbp[(size_t)i * SDMD_NBSLOTS + k];
-    const int o = S.cmap_off[i];
-    A.nrow = S.cmap_off[i + 1] - o;
-    if (A.nrow <= FROW)
-        for (int k = lane; k < A.nrow; k += 64) { wm.row_j[k] = S.cmap_j[o + k]; wm.row_b[k] = S.cmap_bits[o + k]; }
-    if (lane < SDMD_DYNSLOTS) {
-        wm.dyn_j[lane] = S.dynj[(size_t)i * SDMD_DYNSLOTS + lane];
-        wm.dyn_b[lane] = S.dynb[(size_t)i * SDMD_DYNSLOTS + lane];
+    A.x = uni(G(S.x)[i]); A.y = uni(G(S.y)[i]); A.z = uni(G(S.z)[i]);
+    A.vx = uni(G(S.vx)[i]); A.vy = uni(G(S.vy)[i]); A.vz = uni(G(S.vz)[i]);
+    A.counter = uni(G(S.evc)[i]);
+    A.cx = uni(G(S.cx)[i]); A.cy = uni(G(S.cy)[i]); A.cz = uni(G(S.cz)[i]);
+    A.type = uni(G(S.type)[i]); A.hbbond = uni(G(S.hbbond)[i]); A.hbnb = uni(G(S.hbnb)[i]);
+    A.protein = uni(G(S.protein)[i]); A.aanum = uni(G(S.aanum)[i]); A.role = (signed char)uni((int)G(S.role)[i]);
+    A.nbstat = uni(G(S.nbstat)[i]); A.nbnum = uni(G(S.nbnum)[i]);
+    for (int k = 0; k < SDMD_NBSLOTS; k++) A.nbp[k] = uni(G(S.nbp)[(size_t)i * SDMD_NBSLOTS + k]);
+    const int o = uni(G(S.cmap_off)[i]);
+    A.nrow = uni(G(S.cmap_off)[i + 1]) - o;
+    const int bo = uni(G(S.bond_off)[i]);
+    nb = uni(G(S.bond_off)[i + 1]) - bo;
+    // bond entry and its partner (lanes 32.., at most 16 bonds)
+    const int q = lane - 32;
+    bp.j = 0;
+    if (q >= 0 && q < nb && q < 16) {
+        bp.j = G(S.bond_j)[bo + q];
+        bp.lo = G(S.bond_dmin)[bo + q];
+        bp.hi = G(S.bond_dmax)[bo + q];
+        const int j = bp.j;
+        bp.xj = G(S.x)[j]; bp.yj = G(S.y)[j]; bp.zj = G(S.z)[j];
+        bp.vxj = G(S.vx)[j]; bp.vyj = G(S.vy)[j]; bp.vzj = G(S.vz)[j];
+        bp.t0j = G(F.t0)[j];
+        bp.cj = G(S.evc)[j];
     }
+    // connection row (sorted by atom): bitmasks over its span, or staged in
+    // LDS for a binary search when it spans 128 ids or more
+    A.rowlo = A.nrow > 0 ? uni(G(S.cmap_j)[o]) : 0x7fffffff;
+    A.rowhi = A.nrow > 0 ? uni(G(S.cmap_j)[o + A.nrow - 1]) : -1;
+    A.spanok = (A.nrow == 0) || (A.rowhi - A.rowlo < 128);
+    {
+        unsigned long long mb0 = 0, mb1 = 0, mc0 = 0, mc1 = 0;
+        for (int k = lane; k < A.nrow; k += 64) {
+            const int j = G(S.cmap_j)[o + k], b = G(S.cmap_bits)[o + k];
+            if (A.spanok) {
+                const int off = j - A.rowlo;
+                const unsigned long long bit = 1ull << (off & 63);
+                if (b & SDMD_BOND_CONNECT) { if (off >> 6) mb1 |= bit; else mb0 |= bit; }
+                if (b & SDMD_CONSTRAINT_CONNECT) { if (off >> 6) mc1 |= bit; else mc0 |= bit; }
+            } else if (A.nrow <= FROW) {
+                wm.row_j[k] = j;
+                wm.row_b[k] = b;
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            mb0 |= __shfl_xor(mb0, off, 64); mb1 |= __shfl_xor(mb1, off, 64);
+            mc0 |= __shfl_xor(mc0, off, 64); mc1 |= __shfl_xor(mc1, off, 64);
+        }
+        A.mb[0] = (unsigned long long)uni((long long)mb0); A.mb[1] = (unsigned long long)uni((long long)mb1);
+        A.mc[0] = (unsigned long long)uni((long long)mc0); A.mc[1] = (unsigned long long)uni((long long)mc1);
+    }
+    if (lane < SDMD_DYNSLOTS) {
+        wm.dyn_j[lane] = G(S.dynj)[(size_t)i * SDMD_DYNSLOTS + lane];
+        wm.dyn_b[lane] = G(S.dynb)[(size_t)i * SDMD_DYNSLOTS + lane];
+    }
+    // constraint shells (one lane each)
+    {
+        const int co = uni(G(S.con_off)[i]);
+        A.ncon = uni(G(S.con_off)[i + 1]) - co;
+        int ok = 1;
+        if (lane < A.ncon && lane < FCON) {
+            const double cdmin = G(S.con)[co + lane].dmin, cdmax = G(S.con)[co + lane].dmax;
+            const int cstep0 = G(S.con)[co + lane].step0, cnstep = G(S.con)[co + lane].nstep;
+            wm.con_j[lane] = G(S.con_j)[co + lane];
+            wm.con_dmin[lane] = (float)cdmin;
+            wm.con_dmax[lane] = (float)cdmax;
+            wm.con_nstep[lane] = cnstep;
+            unsigned int ez = 0;
+            if (cnstep <= FCONSTEP) {
+                for (int q = 0; q < cnstep; q++) {
+                    wm.con_rad[lane][q] = (float)G(S.sd)[cstep0 + q];
+                    ez |= (G(S.se)[cstep0 + q] == 0 ? 1u : 0u) << q;
+                }
+            } else {
+                ok = 0;
+            }
+            wm.con_ez[lane] = ez;
+        }
+        A.constaged = (A.ncon <= FCON) && !__ballot(!ok);
+    }
+    int cell = 0;
     if (lane < 27) {
         int n1 = A.cx + c_OX[lane], n2 = A.cy + c_OY[lane], n3 = A.cz + c_OZ[lane];
         if (n1 < 0) n1 = S.cellnum[1] - 1; else if (n1 >= S.cellnum[1]) n1 = 0;
         if (n2 < 0) n2 = S.cellnum[2] - 1; else if (n2 >= S.cellnum[2]) n2 = 0;
         if (n3 < 0) n3 = S.cellnum[3] - 1; else if (n3 >= S.cellnum[3]) n3 = 0;
-        wm.cell[lane] = cell_id(S, n1, n2, n3);
+        cell = cell_id(S, n1, n2, n3);
+        wm.cell[lane] = cell;
     }
-    __builtin_amdgcn_wave_barrier();
-    __threadfence_block();
-    int cnt = (lane < 27) ? F.ccnt[wm.cell[lane]] : 0;
+    // own mirror record, as put_record writes it for a state at time t
+    A.s0.x = uni((float)(A.x - A.cx * S.cellsize[1]));
+    A.s0.y = uni((float)(A.y - A.cy * S.cellsize[2]));
+    A.s0.z = uni((float)(A.z - A.cz * S.cellsize[3]));
+    A.s0.w = uni((float)(t - F.Te));
+    A.s1.x = uni((float)A.vx); A.s1.y = uni((float)A.vy); A.s1.z = uni((float)A.vz); A.s1.w = 0.f;
+    const int cnt = (lane < 27) ? G(F.ccnt)[cell] : 0;
     int incl = cnt;
 #pragma unroll
     for (int off = 1; off < 32; off <<= 1) {
@@ -420,7 +584,33 @@ __device__ void fstage(FSys &F, FWave &wm, FAtom &A, int i, double t) {
     if (lane < 27) wm.coff[lane + 1] = incl;
     if (lane == 0) wm.coff[0] = 0;
     __builtin_amdgcn_wave_barrier();
-    __threadfence_block();
+}
+
+// BondTime entry (TimePrediction.c:219-293) from the staged partner state
+__device__ double bond64_pre(const FSys &F, const FAtom &A, const BondPre &bp, int &sub) {
+    const Sys &S = F.s;
+    const int j = bp.j;
+    const double dt = A.t - bp.t0j;
+    Kin k;
+    calc_params_at(S, A.x, A.y, A.z, A.vx, A.vy, A.vz, A.cx, A.cy, A.cz, j, bp.xj + bp.vxj * dt, bp.yj + bp.vyj * dt,
+                   bp.zj + bp.vzj * dt, bp.vxj, bp.vyj, bp.vzj, k);
+    const double lo = bp.lo, hi = bp.hi;
+    double t;
+    if (k.rr - hi > ZERO && k.b >= 0) {
+        sub = SDMD_WELLBOUNC;
+        t = 1e-16 * A.i * j;
+    } else if (!(k.rr - hi > ZERO) && lo - k.rr > ZERO && k.b < 0) {
+        sub = SDMD_CORECOLLI;
+        t = 1e-16 * A.i * j;
+    } else {
+        int s;
+        double d2;
+        if (k.b < 0 && calc_disc(k, lo) > 0) { s = -1; d2 = lo; sub = SDMD_CORECOLLI; }
+        else { s = 1; d2 = hi; sub = SDMD_WELLBOUNC; }
+        t = calc_time(k, s, d2);
+        if (t < 0) t = INFINIT;
+    }
+    return (t > 0) ? t : __builtin_inf();
 }
 
 __device__ __forceinline__ int cand_cell(const FWave &wm, int k) {
@@ -432,116 +622,195 @@ __device__ __forceinline__ int cand_cell(const FWave &wm, int k) {
     return lo;
 }
 
-// screened CollisionTime + the per-wave stats; returns the collision winner
-__device__ void collision_fast(FSys &F, FWave &wm, const FAtom &A, Best &best, long long &nscreen,
+// FP32 screen of candidate k of the staged neighbourhood, from its mirror
+// record (r0, r1) in neighbour cell c
+struct CandS {
+    float t, e;       // screened time and its absolute error bound (t = FINF: no event)
+    int j, bits;
+    bool flag;        // a decision lies in the error band: FP64 decides
+    bool live;        // passed the exclusions and the scan radius
+};
+
+__device__ __forceinline__ void screen_at(const FSys &F, const ScrTab &T, const FWave &wm, const FAtom &A, int c,
+                                          const fvec4 &r0,
+                                          const fvec4 &r1, const fvec4 &s0, const fvec4 &s1, float tA,
+                                          float calib, float cband, CandS &o, int &causal) {
+    const Sys &S = F.s;
+    o.t = FINF; o.e = 0; o.flag = false; o.live = false;
+    const int id = __float_as_int(r1.w);
+    const int j = id >> 8, tj = id & 255;
+    o.j = j;
+    o.bits = 0;
+    if (j == A.i || j < 1 || j > S.N || tj >= NT) return;
+    const int bits = coli_bits(wm, S, A, j);
+    o.bits = bits;
+    if ((bits & SDMD_BOND_CONNECT) || A.hbbond == j || fin_nb_list(A, j)) return;
+    const float dtj = tA - r0.w;
+    const float rx = s0.x - (r0.x + r1.x * dtj) - c_OX[c] * F.L32[0];
+    const float ry = s0.y - (r0.y + r1.y * dtj) - c_OY[c] * F.L32[1];
+    const float rz = s0.z - (r0.z + r1.z * dtj) - c_OZ[c] * F.L32[2];
+    const float rr = rx * rx + ry * ry + rz * rz;
+    if (rr > calib + cband) return;
+    o.live = true;
+    causal += (dtj < 0.f) ? 1 : 0;
+    const float vx = s1.x - r1.x, vy = s1.y - r1.y, vz = s1.z - r1.z;
+    const float b = rx * vx + ry * vy + rz * vz, v2 = vx * vx + vy * vy + vz * vz;
+    const bool con = bits & SDMD_CONSTRAINT_CONNECT;
+    Scr sc;
+    if (con && A.constaged) {
+        int q = -1;
+        for (int k = 0; k < A.ncon; k++)
+            if (wm.con_j[k] == j) { q = k; break; }
+        if (q < 0) {
+            sc.flag = true; sc.t = FINF; sc.e = 0;
+        } else {
+            Shell c;
+            c.dmin = wm.con_dmin[q]; c.dmax = wm.con_dmax[q]; c.nstep = wm.con_nstep[q]; c.ezero = wm.con_ez[q];
+            const float *rad = wm.con_rad[q];
+            sc = screen_shell(c, [&](int k) { return rad[k]; }, true, rr, b, v2);
+        }
+    } else if (con) {
+        const sdmd_pair *pc = nullptr;
+        for (int q = G(S.con_off)[A.i]; q < G(S.con_off)[A.i + 1]; q++)
+            if (G(S.con_j)[q] == j) { pc = &S.con[q]; break; }
+        if (!pc || pc->nstep > 16) {
+            sc.flag = true; sc.t = FINF; sc.e = 0;
+        } else {
+            Shell c;
+            c.dmin = (float)pc->dmin; c.dmax = (float)pc->dmax; c.nstep = pc->nstep; c.ezero = 0;
+            const int s0 = pc->step0;
+            for (int k = 0; k < c.nstep; k++) c.ezero |= (G(S.se)[s0 + k] == 0 ? 1u : 0u) << k;
+            sc = screen_shell(c, [&](int k) { return (float)G(S.sd)[s0 + k]; }, true, rr, b, v2);
+        }
+    } else {
+        const int ri = T.rp[A.type * NT + tj];
+        if (ri < 0) {
+            sc.flag = true; sc.t = FINF; sc.e = 0;
+        } else {
+            const ScrPair &sp = T.pr[ri];
+            Shell c;
+            c.dmin = sp.dmin; c.dmax = sp.dmax; c.nstep = sp.nstep; c.ezero = sp.ezero;
+            const float *rad = T.rad + sp.off;
+            sc = screen_shell(c, [&](int k) { return rad[k]; }, false, rr, b, v2);
+        }
+    }
+    o.t = sc.t; o.e = sc.e; o.flag = sc.flag || (rr > calib - cband);
+}
+
+constexpr int FQ = 4;     // candidates per lane held in registers (256 per wave)
+
+// screened CollisionTime: FP32 screen of the neighbourhood, FP64 re-solve of
+// the marked candidates and of those whose interval reaches the smallest
+// upper bound; the FP64 minimum (first in scan order on ties) competes with
+// the running best under JobAssign's strict '<'
+__device__ __forceinline__ void collision_fast(FSys &F, const ScrTab &T, FWave &wm, const FAtom &A, Best &best, long long &nscreen,
                                long long &nconf, int *err) {
     const Sys &S = F.s;
+    unsigned long long pt = S.prof ? wall_clock64() : 0;
     const int lane = threadIdx.x & 63;
     const int n = wm.coff[27];
     const double calib64 = (S.cutoffr * SDMD_EXTEND_RATIO) * (S.cutoffr * SDMD_EXTEND_RATIO);
     const float calib = (float)calib64;
     const float cband = 2e-5f * calib + 1e-5f;
     const float tA = (float)(A.t - F.Te);
-    // the predicted atom's own mirror (written just before)
-    const size_t selfat = (size_t)cell_id(S, A.cx, A.cy, A.cz) * FCAP + F.slot[A.i];
-    const float4 s0 = F.rec[2 * selfat], s1 = F.rec[2 * selfat + 1];
-    const float L1 = F.L32[0], L2 = F.L32[1], L3 = F.L32[2];
-    // sweep 1: smallest time upper bound
-    float thi = FINF;
-    int my_screen = 0, my_causal = 0;
-    for (int k = lane; k < n; k += 64) {
-        const int c = cand_cell(wm, k);
-        const size_t at = (size_t)wm.cell[c] * FCAP + (k - wm.coff[c]);
-        const float4 r0 = F.rec[2 * at], r1 = F.rec[2 * at + 1];
-        const int id = __float_as_int(r1.w);
-        const int j = id >> 8, tj = id & 255;
-        if (j == A.i || j < 1 || j > S.N || tj >= NT) continue;
-        const int bits = frow_bits(wm, S, A, j);
-        if ((bits & SDMD_BOND_CONNECT) || A.hbbond == j || fin_nb_list(A, j)) continue;
-        const float dtj = tA - r0.w;
-        const float rx = s0.x - (r0.x + r1.x * dtj) - c_OX[c] * L1;
-        const float ry = s0.y - (r0.y + r1.y * dtj) - c_OY[c] * L2;
-        const float rz = s0.z - (r0.z + r1.z * dtj) - c_OZ[c] * L3;
-        const float rr = rx * rx + ry * ry + rz * rz;
-        if (rr > calib + cband) continue;
-        my_causal += (dtj < 0.f) ? 1 : 0;
-        const float vx = s1.x - r1.x, vy = s1.y - r1.y, vz = s1.z - r1.z;
-        const float b = rx * vx + ry * vy + rz * vz, v2 = vx * vx + vy * vy + vz * vz;
-        const bool con = bits & SDMD_CONSTRAINT_CONNECT;
-        const sdmd_pair *pc = nullptr;
-        if (con) {
-            for (int q = S.con_off[A.i]; q < S.con_off[A.i + 1]; q++)
-                if (S.con_j[q] == j) { pc = &S.con[q]; break; }
-        } else {
-            pc = right_pair(S, A.type, tj);
-        }
-        Scr o = screen32(S, pc, con, rr, b, v2);
-        if (rr > calib - cband) o.flag = true;
-        my_screen++;
-        if (!o.flag && o.t < FINF) thi = fminf(thi, o.t + o.e);
-    }
-    thi = wave_min_f(thi);
-    nscreen += wave_sum_i(my_screen);
-    my_causal = wave_sum_i(my_causal);
-    if (my_causal && lane == 0) cnt_add(F.causal, my_causal);
-    // sweep 2: FP64 confirm of the marked candidates and of those that can
-    // reach the bound (and of every candidate when validating)
+    const fvec4 s0 = A.s0, s1 = A.s1;
+    int my_screen = 0, my_causal = 0, my_conf = 0;
     Best mine;
     mine.t = __builtin_inf(); mine.pot = 0; mine.pc = 0; mine.partner = INVALID; mine.type = SDMD_COLI; mine.sub = 0;
     int mink = 0x7fffffff;
     Best full = mine;
     int fullk = 0x7fffffff;
     float myt32 = FINF;
-    int my_conf = 0;
-    for (int k = lane; k < n; k += 64) {
-        const int c = cand_cell(wm, k);
-        const size_t at = (size_t)wm.cell[c] * FCAP + (k - wm.coff[c]);
-        const float4 r0 = F.rec[2 * at], r1 = F.rec[2 * at + 1];
-        const int id = __float_as_int(r1.w);
-        const int j = id >> 8, tj = id & 255;
-        if (j == A.i || j < 1 || j > S.N || tj >= NT) continue;
-        const int bits = frow_bits(wm, S, A, j);
-        if ((bits & SDMD_BOND_CONNECT) || A.hbbond == j || fin_nb_list(A, j)) continue;
-        const float dtj = tA - r0.w;
-        const float rx = s0.x - (r0.x + r1.x * dtj) - c_OX[c] * L1;
-        const float ry = s0.y - (r0.y + r1.y * dtj) - c_OY[c] * L2;
-        const float rz = s0.z - (r0.z + r1.z * dtj) - c_OZ[c] * L3;
-        const float rr = rx * rx + ry * ry + rz * rz;
-        bool need;
-        float t32 = FINF;
-        if (rr > calib + cband) {
-            need = false;
-        } else {
-            const float vx = s1.x - r1.x, vy = s1.y - r1.y, vz = s1.z - r1.z;
-            const float b = rx * vx + ry * vy + rz * vz, v2 = vx * vx + vy * vy + vz * vz;
-            const bool con = bits & SDMD_CONSTRAINT_CONNECT;
-            const sdmd_pair *pc = nullptr;
-            if (con) {
-                for (int q = S.con_off[A.i]; q < S.con_off[A.i + 1]; q++)
-                    if (S.con_j[q] == j) { pc = &S.con[q]; break; }
-            } else {
-                pc = right_pair(S, A.type, tj);
-            }
-            Scr o = screen32(S, pc, con, rr, b, v2);
-            if (rr > calib - cband) o.flag = true;
-            need = o.flag || (o.t < FINF && o.t - o.e <= thi);
-            t32 = o.flag ? FINF : o.t;
+    // FP64 re-solve of candidate k (and the validation scan)
+    auto confirm = [&](int k, const CandS &o, bool need) {
+        double pot;
+        int sub;
+        bool reached;
+        const double t = coli64(F, A, o.j, o.bits, calib64, pot, sub, reached, err);
+        if (need) {
+            my_conf++;
+            if (t < mine.t) { mine.t = t; mine.pot = pot; mine.sub = sub; mine.partner = o.j; mink = k; myt32 = o.t; }
         }
-        if (need || F.validate) {
-            double pot;
-            int sub;
-            bool reached;
-            const double t = coli64(F, A, j, bits, calib64, pot, sub, reached, err);
-            if (need) {
-                my_conf++;
-                if (t < mine.t) {
-                    mine.t = t; mine.pot = pot; mine.sub = sub; mine.partner = j; mink = k; myt32 = t32;
-                }
+        if (t < full.t) { full.t = t; full.pot = pot; full.sub = sub; full.partner = o.j; fullk = k; }
+    };
+    if (n <= 64 * FQ) {
+        // one pass: all record loads issued together, screens kept in registers
+        fvec4 r0[FQ], r1[FQ];
+        int cq[FQ];
+#pragma unroll
+        for (int q = 0; q < FQ; q++) {
+            const int k = lane + 64 * q;
+            cq[q] = 0;
+            if (k < n) {
+                const int c = cand_cell(wm, k);
+                const size_t at = (size_t)wm.cell[c] * FCAP + (k - wm.coff[c]);
+                cq[q] = c;
+                r0[q] = G(F.rec)[2 * at];
+                r1[q] = G(F.rec)[2 * at + 1];
             }
-            if (t < full.t) { full.t = t; full.pot = pot; full.sub = sub; full.partner = j; fullk = k; }
+        }
+        CandS o[FQ];
+        float thi = FINF;
+#pragma unroll
+        for (int q = 0; q < FQ; q++) {
+            o[q].live = false; o[q].t = FINF; o[q].flag = false;
+            if (lane + 64 * q < n) {
+                screen_at(F, T, wm, A, cq[q], r0[q], r1[q], s0, s1, tA, calib, cband, o[q], my_causal);
+                my_screen += o[q].live;
+                if (o[q].live && !o[q].flag && o[q].t < FINF) thi = fminf(thi, o[q].t + o[q].e);
+            }
+        }
+        thi = wave_min_f(thi);
+        PROF_MARK(S, FPH_SCREEN, pt);
+        // one copy of the FP64 re-solve, looped over the marked slots
+        unsigned needm = 0, valm = 0;
+#pragma unroll
+        for (int q = 0; q < FQ; q++) {
+            const bool need = o[q].live && (o[q].flag || (o[q].t < FINF && o[q].t - o[q].e <= thi));
+            needm |= (need ? 1u : 0u) << q;
+            valm |= ((o[q].live && F.validate) ? 1u : 0u) << q;
+        }
+        unsigned todo = needm | valm;
+        while (todo) {
+            const int q = __builtin_ctz(todo);
+            todo &= todo - 1;
+            CandS oq;
+            oq.j = o[0].j; oq.bits = o[0].bits; oq.t = o[0].t;
+#pragma unroll
+            for (int r = 1; r < FQ; r++)
+                if (q == r) { oq.j = o[r].j; oq.bits = o[r].bits; oq.t = o[r].t; }
+            confirm(lane + 64 * q, oq, (needm >> q) & 1);
+        }
+    } else {
+        // large neighbourhoods: bound first, then re-screen and confirm
+        float thi = FINF;
+        for (int k = lane; k < n; k += 64) {
+            const int c = cand_cell(wm, k);
+            const size_t at = (size_t)wm.cell[c] * FCAP + (k - wm.coff[c]);
+            CandS o;
+            const fvec4 q0 = G(F.rec)[2 * at], q1 = G(F.rec)[2 * at + 1];
+            screen_at(F, T, wm, A, c, q0, q1, s0, s1, tA, calib, cband, o, my_causal);
+            my_screen += o.live;
+            if (o.live && !o.flag && o.t < FINF) thi = fminf(thi, o.t + o.e);
+        }
+        thi = wave_min_f(thi);
+        int dummy = 0;
+        for (int k = lane; k < n; k += 64) {
+            const int c = cand_cell(wm, k);
+            const size_t at = (size_t)wm.cell[c] * FCAP + (k - wm.coff[c]);
+            CandS o;
+            const fvec4 q0 = G(F.rec)[2 * at], q1 = G(F.rec)[2 * at + 1];
+            screen_at(F, T, wm, A, c, q0, q1, s0, s1, tA, calib, cband, o, dummy);
+            if (!o.live) continue;
+            const bool need = o.flag || (o.t < FINF && o.t - o.e <= thi);
+            if (need || F.validate) confirm(k, o, need);
         }
     }
+    PROF_MARK(S, FPH_CONFIRM, pt);
+    nscreen += wave_sum_i(my_screen);
     nconf += wave_sum_i(my_conf);
+    my_causal = wave_sum_i(my_causal);
+    if (my_causal && lane == 0) cnt_add(F.causal, my_causal);
     // wave argmin (first minimal candidate)
     double tw = mine.t;
     int kw = mink;
@@ -575,7 +844,7 @@ __device__ void collision_fast(FSys &F, FWave &wm, const FAtom &A, Best &best, l
     if (win.t < __builtin_inf() && win.partner > 0) {
         if (win.t > 0 && best.t > win.t) {
             best.t = win.t; best.pot = win.pot; best.sub = win.sub; best.type = SDMD_COLI;
-            best.partner = win.partner; best.pc = S.evc[win.partner];
+            best.partner = win.partner; best.pc = G(S.evc)[win.partner];
         }
     }
 }
@@ -583,8 +852,8 @@ __device__ void collision_fast(FSys &F, FWave &wm, const FAtom &A, Best &best, l
 // one BondTime entry (TimePrediction.c:219-293) at local time
 __device__ double bond64(const FSys &F, const FAtom &A, int e, int &sub) {
     const Sys &S = F.s;
-    const int j = S.bond_j[e];
-    const double lo = S.bond_dmin[e], hi = S.bond_dmax[e];
+    const int j = G(S.bond_j)[e];
+    const double lo = G(S.bond_dmin)[e], hi = G(S.bond_dmax)[e];
     Kin k;
     kin_local(F, A, j, k);
     double t;
@@ -610,13 +879,13 @@ __device__ double hbfree64(const FSys &F, const FAtom &A, int j, double calib, d
     const Sys &S = F.s;
     reached = false;
     pot = 0;
-    const signed char ri = A.role, rj = S.role[j];
+    const signed char ri = A.role, rj = G(S.role)[j];
     const bool pairok = (ri == 'A' && rj == 'D') || (ri == 'D' && rj == 'A') ||
-                        (ri == 'S' && S.type[j] == 2) || (rj == 'S' && A.type == 2) || (ri == 'S' && rj == 'S');
-    int sep = A.aanum - S.aanum[j];
+                        (ri == 'S' && G(S.type)[j] == 2) || (rj == 'S' && A.type == 2) || (ri == 'S' && rj == 'S');
+    int sep = A.aanum - G(S.aanum)[j];
     if (sep < 0) sep = -sep;
-    const int pj = S.protein[j];
-    if (!(j != A.i && pairok && S.hbbond[j] == 0 && ((A.protein == pj && sep >= 4) || A.protein != pj)))
+    const int pj = G(S.protein)[j];
+    if (!(j != A.i && pairok && G(S.hbbond)[j] == 0 && ((A.protein == pj && sep >= 4) || A.protein != pj)))
         return __builtin_inf();
     Kin k;
     kin_local(F, A, j, k);
@@ -634,39 +903,45 @@ __device__ double hbfree64(const FSys &F, const FAtom &A, int j, double calib, d
 }
 
 // Predict for atom i at time t (its state is at t); writes the event record
-__device__ void predict_fast(FSys &F, FWave &wm, int i, double t, long long &nscreen, long long &nconf,
-                             long long &nfp64, int *err) {
+__device__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm, int i, double t, WStats &ws, int *err) {
     Sys &S = F.s;
     const int lane = threadIdx.x & 63;
+    long long &nscreen = ws.screen, &nconf = ws.conf, &nfp64 = ws.fp64;
+    unsigned long long pt = S.prof ? wall_clock64() : 0;
     FAtom A;
-    fstage(F, wm, A, i, t);
-    if (A.nrow > FROW && lane == 0) { /* falls back to global row lookups */ }
-    Best best;
-    best.t = INFINIT; best.pot = 0; best.pc = 0; best.partner = INVALID; best.type = SDMD_INVD; best.sub = 0;
-    // BondTime: one lane per bond, first minimal wins
+    BondPre bp;
+    int nb = 0;
+    fstage(F, wm, A, i, t, bp, nb);
+    ws.cand += wm.coff[27] - 1;
+    PROF_MARK(S, FPH_STAGE, pt);
+    if (nb > 16 && lane == 0) *err = ERR_CELLCAP;      // BondTime's connect[16], TimePrediction.c:188
+    Best init;
+    init.t = INFINIT; init.pot = 0; init.pc = 0; init.partner = INVALID; init.type = SDMD_INVD; init.sub = 0;
+    // CollisionTime (screened), against no running best yet
+    Best cw = init;
+    cw.t = __builtin_inf();
+    collision_fast(F, T, wm, A, cw, nscreen, nconf, err);
+    if (S.prof) pt = wall_clock64();
+    // BondTime: one lane per bond (lanes 32..), first minimal wins
+    Best best = init;
     {
-        const int o = S.bond_off[i], nb = S.bond_off[i + 1] - o;
-        Best run = best;
-        for (int base = 0; base < nb; base += 64) {
-            Best mine = best;
-            mine.t = __builtin_inf();
-            const int q = base + lane;
-            if (q < nb) {
-                int sub;
-                const double tb = bond64(F, A, o + q, sub);
-                mine.t = tb; mine.sub = sub; mine.pot = 0; mine.type = SDMD_BOND; mine.partner = S.bond_j[o + q];
-                mine.pc = S.evc[mine.partner];
-            }
-            Best out;
-            int win;
-            wave_best(mine, q, run, out, win);
-            run = out;
+        Best mine = init;
+        mine.t = __builtin_inf();
+        const int q = lane - 32;
+        if (bp.j > 0) {
+            int sub;
+            mine.t = bond64_pre(F, A, bp, sub);
+            mine.sub = sub; mine.pot = 0; mine.type = SDMD_BOND; mine.partner = bp.j; mine.pc = bp.cj;
         }
+        Best out;
+        int win;
+        wave_best(mine, q, best, out, win);
+        best = out;
         nfp64 += (lane == 0) ? nb : 0;
-        best = run;
     }
-    // CollisionTime (screened)
-    collision_fast(F, wm, A, best, nscreen, nconf, err);
+    // JobAssign order: bonds first, then the collision winner if strictly earlier
+    if (cw.partner > 0 && cw.t > 0 && best.t > cw.t) best = cw;
+    PROF_MARK(S, FPH_BOND, pt);
     // PBCandCrossCellTime, TimePrediction.c:598-632
     {
         double t_ev = INFINIT;
@@ -695,9 +970,9 @@ __device__ void predict_fast(FSys &F, FWave &wm, int i, double t, long long &nsc
     }
     // HBTime head, TimePrediction.c:309-330
     const double calib = (S.cutoffr * SDMD_EXTEND_RATIO) * (S.cutoffr * SDMD_EXTEND_RATIO);
-    const int hbit = S.hbitype[i];
+    const int hbit = G(S.hbitype)[i];
     __builtin_amdgcn_wave_barrier();
-    if (lane == 0) S.hbitype[i] = SDMD_NOEVENT;
+    if (lane == 0) G(S.hbitype)[i] = SDMD_NOEVENT;
     if (hbit != SDMD_HBSKIP) {
         if (A.hbbond > 0) {
             // bonded HB partner, TimePrediction.c:330-383
@@ -729,7 +1004,7 @@ __device__ void predict_fast(FSys &F, FWave &wm, int i, double t, long long &nsc
             for (int k = lane; k < n; k += 64) {
                 const int c = cand_cell(wm, k);
                 const size_t at = (size_t)wm.cell[c] * FCAP + (k - wm.coff[c]);
-                const int j = __float_as_int(F.rec[2 * at + 1].w) >> 8;
+                const int j = __float_as_int(G(F.rec)[2 * at + 1].w) >> 8;
                 if (j < 1 || j > S.N) continue;
                 double pot;
                 bool reached;
@@ -760,8 +1035,8 @@ __device__ void predict_fast(FSys &F, FWave &wm, int i, double t, long long &nsc
             kin_local(F, A, j, k);
             int host, nb, partner;
             const bool ads = (A.role == 'A' || A.role == 'D' || A.role == 'S');
-            if (ads && A.hbbond && S.hbnb[A.hbbond] == j) { host = i; nb = j; partner = A.hbbond; }
-            else { host = j; nb = i; partner = S.hbbond[j]; }
+            if (ads && A.hbbond && G(S.hbnb)[A.hbbond] == j) { host = i; nb = j; partner = A.hbbond; }
+            else { host = j; nb = i; partner = G(S.hbbond)[j]; }
             FP f;
             const int cs = find_pair(S, nb, host, K_NEIGHBOR, k.b, k.rr, f, partner, 0, err);
             int s, sub;
@@ -783,45 +1058,46 @@ __device__ void predict_fast(FSys &F, FWave &wm, int i, double t, long long &nsc
             }
         }
     }
+    PROF_MARK(S, FPH_OTHER, pt);
     if (lane == 0) {
-        S.evt[i] = t + best.t;
-        S.evpot[i] = best.pot;
-        S.evpc[i] = best.pc;
-        S.evp[i] = best.partner;
-        S.evtype[i] = best.type;
-        S.evsub[i] = best.sub;
-        S.nbnum[i] = A.nbnum;
+        G(S.evt)[i] = t + best.t;
+        G(S.evpot)[i] = best.pot;
+        G(S.evpc)[i] = best.pc;
+        G(S.evp)[i] = best.partner;
+        G(S.evtype)[i] = best.type;
+        G(S.evsub)[i] = best.sub;
+        G(S.nbnum)[i] = A.nbnum;
     }
 }
 
 // ---------------------------------------------------------------- events
 __device__ void cell_remove(FSys &F, int a, int cell, int *err) {
-    const int q = F.slot[a];
-    const int last = F.ccnt[cell] - 1;
+    const int q = G(F.slot)[a];
+    const int last = G(F.ccnt)[cell] - 1;
     if (q < 0 || q > last || last >= FCAP) { *err = FERR_SLOT; return; }
     const size_t base = (size_t)cell * FCAP;
     if (q != last) {
-        const int m = F.cmem[base + last];
-        F.cmem[base + q] = m;
-        F.rec[2 * (base + q)] = F.rec[2 * (base + last)];
-        F.rec[2 * (base + q) + 1] = F.rec[2 * (base + last) + 1];
-        F.slot[m] = q;
+        const int m = G(F.cmem)[base + last];
+        G(F.cmem)[base + q] = m;
+        G(F.rec)[2 * (base + q)] = G(F.rec)[2 * (base + last)];
+        G(F.rec)[2 * (base + q) + 1] = G(F.rec)[2 * (base + last) + 1];
+        G(F.slot)[m] = q;
     }
-    F.ccnt[cell] = last;
+    G(F.ccnt)[cell] = last;
 }
 
 __device__ void cell_insert(FSys &F, int a, int cell, int *err) {
-    const int q = F.ccnt[cell];
+    const int q = G(F.ccnt)[cell];
     if (q >= FCAP) { *err = ERR_CELLCAP; return; }
-    F.cmem[(size_t)cell * FCAP + q] = a;
-    F.slot[a] = q;
-    F.ccnt[cell] = q + 1;
+    G(F.cmem)[(size_t)cell * FCAP + q] = a;
+    G(F.slot)[a] = q;
+    G(F.ccnt)[cell] = q + 1;
 }
 
 // PBCandCrossCellEvent + PBC("part") (Events.c:207-272) on the cell arrays
 __device__ void pcc_fast(FSys &F, int i, int *err) {
     Sys &S = F.s;
-    const int old[3] = {S.cx[i], S.cy[i], S.cz[i]};
+    const int old[3] = {G(S.cx)[i], G(S.cy)[i], G(S.cz)[i]};
     double *c[3] = {S.x, S.y, S.z};
     int *ci[3] = {S.cx, S.cy, S.cz};
     for (int n = 0; n < 3; n++) {
@@ -856,37 +1132,37 @@ __device__ void thermostat_fast(FSys &F, int i) {
     for (int n = 0; n < 3; n++) {
         int guard = 0;
         do {
-            const double variance = S.T / S.mass[i];
+            const double variance = S.T / G(S.mass)[i];
             sp[n] = fast_normal(F, i) * sqrt(variance) + 0;
         } while (sp[n] == 0 && ++guard < 8);
     }
-    S.vx[i] = sp[0]; S.vy[i] = sp[1]; S.vz[i] = sp[2];
+    G(S.vx)[i] = sp[0]; G(S.vy)[i] = sp[1]; G(S.vz)[i] = sp[2];
 }
 
 // execute one popped event on lane 0 (DoEvent, Events.c:25-114, with the
 // counter increments of SGThread.c:100-104); returns the renew count
 __device__ int exec_fast(FSys &F, int a, double t, int *renew, int *err) {
     Sys &S = F.s;
-    const int p = S.evp[a];
+    const int p = G(S.evp)[a];
     const int partner = (p > 0 && p <= S.N) ? p : 0;
     advance(F, a, t);
     renew[0] = a;
-    if (partner && S.evpc[a] < S.evc[partner]) {          // HazardCheck, SGThread.c:215-235
+    if (partner && G(S.evpc)[a] < G(S.evc)[partner]) {          // HazardCheck, SGThread.c:215-235
         cnt_add(S.recal, 1);
         return 1;
     }
     if (partner) {
-        if (F.t0[partner] > t) cnt_add(F.causal, 1);
+        if (G(F.t0)[partner] > t) cnt_add(F.causal, 1);
         advance(F, partner, t);
     }
-    const int ty = S.evtype[a];
+    const int ty = G(S.evtype)[a];
     const PosLocal pos{F, t};
     switch (ty) {
     case SDMD_COLI: case SDMD_BOND: case SDMD_HBNE:
         interaction_event(S, pos, a, partner, 0, 0, err);
         break;
     case SDMD_HB: {
-        const int nbi = S.hbnb[a], nbj = S.hbnb[partner];
+        const int nbi = G(S.hbnb)[a], nbj = G(S.hbnb)[partner];
         interaction_event(S, pos, a, partner, nbi, nbj, err);
         hb_event(S, a, partner, nbi, nbj, err);
         break;
@@ -897,38 +1173,57 @@ __device__ int exec_fast(FSys &F, int a, double t, int *renew, int *err) {
     }
     if (ty >= 0 && ty < 16) cnt_add(S.per_type[ty], 1);
     cnt_add(S.frame, 1);
-    S.evc[a]++;
-    if (partner) S.evc[partner]++;
+    G(S.evc)[a]++;
+    if (partner) G(S.evc)[partner]++;
     if (partner) { renew[1] = partner; return 2; }
     return 1;
 }
 
 // ---------------------------------------------------------------- tree blocks
-// block b covers atoms 64b+1 .. 64b+64; returns (min, atom, second min)
-__device__ void block_scan(const FSys &F, int b, double &t1, int &a1, double &t2) {
+// Block b covers atoms 64b+1 .. 64b+64.  Returns its earliest event (t1,
+// a1), the next one (t2, a2) and the third time t3.  An atom whose own event
+// names a1 (or a2) as partner while a1's (a2's) names it back is the same
+// pair event seen from the other side: the earlier one's execution
+// re-predicts it, and the earlier one's independence test covers its atoms,
+// so it is skipped when ranking.
+__device__ void block_scan(const FSys &F, int b, double &t1, int &a1, double &t2, int &a2, double &t3) {
     const int lane = threadIdx.x & 63;
     const int a = b * 64 + lane + 1;
-    double t = (a <= F.s.N) ? F.s.evt[a] : __builtin_inf();
-    int k = (a <= F.s.N) ? a : 0x7fffffff;
+    const bool in = a <= F.s.N;
+    const double t = in ? G(F.s.evt)[a] : __builtin_inf();
+    const int p = in ? G(F.s.evp)[a] : 0;
+    const int k = in ? a : 0x7fffffff;
     double m = t;
     int km = k;
     wave_argmin(m, km);
-    double u = (k == km) ? __builtin_inf() : t;
+    const int pm = __shfl(p, (km - 1 - b * 64) & 63, 64);
+    const bool out1 = (k == km) || (k == pm && p == km);
+    double u = out1 ? __builtin_inf() : t;
+    int ku = out1 ? 0x7fffffff : k;
+    wave_argmin(u, ku);
+    const int pu = __shfl(p, (ku - 1 - b * 64) & 63, 64);
+    const bool out2 = out1 || (k == ku) || (k == pu && p == ku);
+    double w = out2 ? __builtin_inf() : t;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) u = fmin(u, __shfl_xor(u, off, 64));
-    t1 = m; a1 = km; t2 = u;
+    for (int off = 32; off > 0; off >>= 1) w = fmin(w, __shfl_xor(w, off, 64));
+    t1 = m; a1 = km; t2 = u; a2 = ku; t3 = w;
 }
 
 // ---------------------------------------------------------------- shared memory
 struct FShared {
-    double b1[FMAXBLK], b2[FMAXBLK];
-    int ba[FMAXBLK];
-    double red_t[FMAXW], red_t1[FMAXW];
+    double b1[FMAXBLK], b2[FMAXBLK], b3[FMAXBLK];
+    int ba[FMAXBLK], bb[FMAXBLK];
+    double red_t[FMAXW], red_t1[FMAXW], red_t3[FMAXW];
     int red_a[FMAXW], red_n[FMAXW];
     double sel_t[FSEL];
-    int sel_a[FSEL];
-    int nsel, nex, ndirty, err, stop;
+    int sel_a[FSEL], sel_ca[FSEL], sel_cp[FSEL], sel_pcc[FSEL];
+    unsigned long long conflict;
+    int nsort;
+    int nsel, nex, ndirty, nren, err, stop, drain;
+    long long drain_end;
     int ex_a[FSEL];
+    int ren_a[2 * FSEL];
+    double ren_t[2 * FSEL];
     int dirty[4 * FSEL];
     FWave w[FMAXW];
 };
@@ -956,29 +1251,36 @@ __device__ void rebase(FSys &F, double Te) {
     __syncthreads();
     const Sys &S = F.s;
     for (int a = threadIdx.x + 1; a <= S.N; a += 64 * W) {
-        const int cx = S.cx[a], cy = S.cy[a], cz = S.cz[a];
-        const size_t at = (size_t)cell_id(S, cx, cy, cz) * FCAP + F.slot[a];
-        const double dt = Te - F.t0[a];
-        float4 r0;
-        r0.x = (float)((S.x[a] + S.vx[a] * dt) - cx * S.cellsize[1]);
-        r0.y = (float)((S.y[a] + S.vy[a] * dt) - cy * S.cellsize[2]);
-        r0.z = (float)((S.z[a] + S.vz[a] * dt) - cz * S.cellsize[3]);
+        const int cx = G(S.cx)[a], cy = G(S.cy)[a], cz = G(S.cz)[a];
+        const size_t at = (size_t)cell_id(S, cx, cy, cz) * FCAP + G(F.slot)[a];
+        const double dt = Te - G(F.t0)[a];
+        fvec4 r0;
+        r0.x = (float)((G(S.x)[a] + G(S.vx)[a] * dt) - cx * S.cellsize[1]);
+        r0.y = (float)((G(S.y)[a] + G(S.vy)[a] * dt) - cy * S.cellsize[2]);
+        r0.z = (float)((G(S.z)[a] + G(S.vz)[a] * dt) - cz * S.cellsize[3]);
         r0.w = 0.f;
-        F.rec[2 * at] = r0;
+        G(F.rec)[2 * at] = r0;
     }
     __syncthreads();
 }
 
-// stats accumulated per wave, added to the system at the end
-struct WStats { long long screen, conf, fp64; };
 
 // ---------------------------------------------------------------- kernels
 // bulk Predict(1..N) at the system time and the block minima
 template <int W>
+__device__ __forceinline__ void tab_load(ScrTab &dst, const void *src) {
+    const int n = (int)(sizeof(ScrTab) / sizeof(int));
+    for (int k = threadIdx.x; k < n; k += 64 * W) ((int *)&dst)[k] = ((const int *)src)[k];
+    __syncthreads();
+}
+
+template <int W>
 __global__ __launch_bounds__(64 * W) void k_fast_prepare(FSys *systems) {
     __shared__ FSys Fl;
     __shared__ FShared sh;
+    __shared__ ScrTab tab;
     fsys_load<W>(Fl, &systems[blockIdx.x]);
+    tab_load<W>(tab, Fl.scrtab);
     FSys &F = Fl;
     Sys &S = F.s;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -986,24 +1288,25 @@ __global__ __launch_bounds__(64 * W) void k_fast_prepare(FSys *systems) {
     __syncthreads();
     for (int a = tid + 1; a <= S.N; a += 64 * W) put_record(F, a);
     __syncthreads();
-    WStats ws = {0, 0, 0};
+    WStats ws = {0, 0, 0, 0};
     int err = 0;
     for (int i = wave + 1; i <= S.N; i += W) {
-        predict_fast(F, sh.w[wave], i, F.t0[i], ws.screen, ws.conf, ws.fp64, &err);
-        if (err) break;
+        predict_fast(F, tab, sh.w[wave], i, G(F.t0)[i], ws, &err);
+        if (__ballot(err != 0)) break;              // uniform across the wave
     }
     if (err) atomicCAS(&sh.err, 0, err);
     __syncthreads();
     for (int b = wave; b < F.nblk; b += W) {
-        double t1, t2;
-        int a1;
-        block_scan(F, b, t1, a1, t2);
-        if (lane == 0) { F.b1[b] = t1; F.ba[b] = a1; F.b2[b] = t2; }
+        double t1, t2, t3;
+        int a1, a2;
+        block_scan(F, b, t1, a1, t2, a2, t3);
+        if (lane == 0) { G(F.b1)[b] = t1; G(F.ba)[b] = a1; G(F.b2)[b] = t2; G(F.bb)[b] = a2; G(F.b3)[b] = t3; }
     }
     if (lane == 0) {
         atomicAdd((unsigned long long *)&F.screens, (unsigned long long)ws.screen);
         atomicAdd((unsigned long long *)&F.confirms, (unsigned long long)ws.conf);
         atomicAdd((unsigned long long *)&S.pair_evals, (unsigned long long)(ws.fp64 + ws.conf));
+        atomicAdd((unsigned long long *)&S.candidates, (unsigned long long)ws.cand);
     }
     __syncthreads();
     if (tid == 0 && sh.err) S.error = sh.err;
@@ -1016,41 +1319,65 @@ template <int W>
 __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long npops, long long budget) {
     __shared__ FSys Fl;
     __shared__ FShared sh;
+    __shared__ ScrTab tab;
     fsys_load<W>(Fl, &systems[blockIdx.x]);
+    tab_load<W>(tab, Fl.scrtab);
     FSys &F = Fl;
     Sys &S = F.s;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int nblk = F.nblk;
-    for (int b = tid; b < nblk; b += 64 * W) { sh.b1[b] = F.b1[b]; sh.b2[b] = F.b2[b]; sh.ba[b] = F.ba[b]; }
-    if (tid == 0) { sh.err = S.error; sh.stop = 0; }
+    for (int b = tid; b < nblk; b += 64 * W) {
+        sh.b1[b] = G(F.b1)[b]; sh.b2[b] = G(F.b2)[b]; sh.b3[b] = G(F.b3)[b];
+        sh.ba[b] = G(F.ba)[b]; sh.bb[b] = G(F.bb)[b];
+    }
+    if (tid == 0) { sh.err = S.error; sh.stop = 0; sh.drain = 0; }
     __syncthreads();
     const long long t_start = (long long)wall_clock64();
     const long long pops0 = S.pops;
-    WStats ws = {0, 0, 0};
+    WStats ws = {0, 0, 0, 0};
     const int cap = F.max_batch > 0 ? min(F.max_batch, FSEL) : FSEL;
+    unsigned long long pt = S.prof ? wall_clock64() : 0;
     while (true) {
-        // ---- selection: every event earlier than the smallest second minimum
-        double m2 = __builtin_inf(), m1 = __builtin_inf();
+        // ---- selection: every event earlier than the smallest third time of
+        // any block (each block's two earliest), or, when that set is too
+        // large, earlier than the smallest second time (each block's earliest)
+        double m3 = __builtin_inf(), m2 = __builtin_inf(), m1 = __builtin_inf();
         int a1 = 0x7fffffff;
         for (int b = tid; b < nblk; b += 64 * W) {
+            m3 = fmin(m3, sh.b3[b]);
             m2 = fmin(m2, sh.b2[b]);
             const double t = sh.b1[b];
             if (t < m1 || (t == m1 && sh.ba[b] < a1)) { m1 = t; a1 = sh.ba[b]; }
         }
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) m2 = fmin(m2, __shfl_xor(m2, off, 64));
+        for (int off = 32; off > 0; off >>= 1) {
+            m3 = fmin(m3, __shfl_xor(m3, off, 64));
+            m2 = fmin(m2, __shfl_xor(m2, off, 64));
+        }
         wave_argmin(m1, a1);
-        if (lane == 0) { sh.red_t[wave] = m2; sh.red_t1[wave] = m1; sh.red_a[wave] = a1; }
+        if (lane == 0) { sh.red_t[wave] = m2; sh.red_t3[wave] = m3; sh.red_t1[wave] = m1; sh.red_a[wave] = a1; }
         __syncthreads();
-        double theta = sh.red_t[0], tmin = sh.red_t1[0];
+        const double tcap = nextafter(F.t_stop, __builtin_inf());
+        double theta = fmin(sh.red_t[0], tcap), theta3 = fmin(sh.red_t3[0], tcap), tmin = sh.red_t1[0];
         int amin = sh.red_a[0];
         for (int w = 1; w < W; w++) {
             theta = fmin(theta, sh.red_t[w]);
+            theta3 = fmin(theta3, sh.red_t3[w]);
             if (sh.red_t1[w] < tmin || (sh.red_t1[w] == tmin && sh.red_a[w] < amin)) { tmin = sh.red_t1[w]; amin = sh.red_a[w]; }
         }
+        PROF_MARK(S, FPH_SCAN, pt);
         if (tid == 0) {
-            if (sh.err || S.pops - pops0 >= npops || ((long long)wall_clock64() - t_start) > budget ||
-                !(tmin < INFINIT) || amin < 1 || amin > S.N)
+            // Once the pop or time budget is spent, drain: execute what is
+            // still pending up to the latest executed time, so that every
+            // atom's local time is at most that time and every pending event
+            // is later -- the state is then a snapshot at one time.
+            if (!sh.drain && (S.pops - pops0 >= npops || ((long long)wall_clock64() - t_start) > budget)) {
+                sh.drain = 1;
+                if (F.tlast < F.t_stop) F.t_stop = F.tlast;
+                sh.drain_end = S.pops + FDRAIN;
+            }
+            if (sh.err || !(tmin < INFINIT) || !(tmin <= F.t_stop) || amin < 1 || amin > S.N ||
+                (sh.drain && S.pops >= sh.drain_end))
                 sh.stop = 1;
             sh.nsel = 0;
         }
@@ -1058,15 +1385,30 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
         if (sh.stop) break;
         if (tmin - F.Te > F.rebase_dt) rebase<W>(F, tmin);
         // collect the prefix set
-        for (int b0 = 0; b0 < nblk; b0 += 64 * W) {
-            const int b = b0 + tid;
-            const bool in = b < nblk && sh.b1[b] < theta;
-            if (in) {
+        for (int b = tid; b < nblk; b += 64 * W) {
+            if (sh.b1[b] < theta3) {
                 const int slotk = atomicAdd(&sh.nsel, 1);
                 if (slotk < FSEL) { sh.sel_t[slotk] = sh.b1[b]; sh.sel_a[slotk] = sh.ba[b]; }
             }
+            if (sh.b2[b] < theta3) {
+                const int slotk = atomicAdd(&sh.nsel, 1);
+                if (slotk < FSEL) { sh.sel_t[slotk] = sh.b2[b]; sh.sel_a[slotk] = sh.bb[b]; }
+            }
         }
         __syncthreads();
+        if (sh.nsel > FSEL) {
+            __syncthreads();
+            if (tid == 0) sh.nsel = 0;
+            __syncthreads();
+            for (int b = tid; b < nblk; b += 64 * W) {
+                if (sh.b1[b] < theta) {
+                    const int slotk = atomicAdd(&sh.nsel, 1);
+                    if (slotk < FSEL) { sh.sel_t[slotk] = sh.b1[b]; sh.sel_a[slotk] = sh.ba[b]; }
+                }
+            }
+            __syncthreads();
+        }
+        PROF_MARK(S, FPH_COLLECT, pt);
         if (wave == 0) {
             int nsel = sh.nsel;
             bool single = (nsel == 0 || nsel > FSEL);
@@ -1092,33 +1434,57 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
                     if (take_o) { t = to; a = ao; }
                 }
             }
-            // independence: cells of target and partner against every earlier event
-            int p = 0, cxa = 0, cya = 0, cza = 0, cxp = 0, cyp = 0, czp = 0, pcc = 0;
+            // cells of each event's target and partner, for the independence test
             const bool valid = lane < nsel && a >= 1 && a <= S.N;
+            int ca = 0, cp = 0, pcc = 0;
             if (valid) {
-                cxa = S.cx[a]; cya = S.cy[a]; cza = S.cz[a];
-                p = S.evp[a];
-                if (p > 0 && p <= S.N) { cxp = S.cx[p]; cyp = S.cy[p]; czp = S.cz[p]; }
-                else { p = 0; cxp = cxa; cyp = cya; czp = cza; }
-                pcc = (S.evtype[a] == SDMD_PCC);
+                ca = (G(S.cx)[a] << 20) | (G(S.cy)[a] << 10) | G(S.cz)[a];
+                const int p = G(S.evp)[a];
+                cp = (p > 0 && p <= S.N) ? ((G(S.cx)[p] << 20) | (G(S.cy)[p] << 10) | G(S.cz)[p]) : ca;
+                pcc = (G(S.evtype)[a] == SDMD_PCC);
             }
+            if (lane < 64) { sh.sel_a[lane] = a; sh.sel_ca[lane] = ca; sh.sel_cp[lane] = cp; sh.sel_pcc[lane] = pcc; }
+            if (lane == 0) { sh.nsort = nsel; sh.conflict = 0; }
+            if (lane == 0) {
+                F.batches++;
+                if (single) F.single++;
+                sh.ndirty = 0;
+                sh.nren = 0;
+            }
+        }
+        __syncthreads();
+        // independence: every pair (l < k) of the sorted set, spread over the
+        // workgroup; event k conflicts when an atom of an earlier event l lies
+        // within D cells (Chebyshev, periodic) of one of its atoms
+        {
+            const int ns = sh.nsort;
+            const int npair = ns * (ns - 1) / 2;
             const int n1 = S.cellnum[1], n2 = S.cellnum[2], n3 = S.cellnum[3];
-            bool ok = valid;
-            for (int l = 0; l < nsel - 1 && l < 63; l++) {
-                const int ox = __shfl(cxa, l, 64), oy = __shfl(cya, l, 64), oz = __shfl(cza, l, 64);
-                const int qx = __shfl(cxp, l, 64), qy = __shfl(cyp, l, 64), qz = __shfl(czp, l, 64);
-                const int opcc = __shfl(pcc, l, 64);
-                if (l < lane && ok) {
-                    const int D = 3 + ((pcc | opcc) ? 1 : 0);
-                    auto far = [&](int x1, int y1, int z1, int x2, int y2, int z2) {
-                        int dx = abs(x1 - x2), dy = abs(y1 - y2), dz = abs(z1 - z2);
-                        dx = min(dx, n1 - dx); dy = min(dy, n2 - dy); dz = min(dz, n3 - dz);
-                        return max(dx, max(dy, dz)) >= D;
-                    };
-                    ok = far(cxa, cya, cza, ox, oy, oz) && far(cxa, cya, cza, qx, qy, qz) &&
-                         far(cxp, cyp, czp, ox, oy, oz) && far(cxp, cyp, czp, qx, qy, qz);
-                }
+            unsigned long long mine = 0;
+            for (int q = tid; q < npair; q += 64 * W) {
+                int k = (int)((1.f + sqrtf(1.f + 8.f * (float)q)) * 0.5f);
+                while (k * (k - 1) / 2 > q) k--;
+                while ((k + 1) * k / 2 <= q) k++;
+                const int l = q - k * (k - 1) / 2;
+                const int D = 3 + ((sh.sel_pcc[k] | sh.sel_pcc[l]) ? 1 : 0);
+                auto far = [&](int u, int v) {
+                    int dx = abs((u >> 20) - (v >> 20)), dy = abs(((u >> 10) & 1023) - ((v >> 10) & 1023)),
+                        dz = abs((u & 1023) - (v & 1023));
+                    dx = min(dx, n1 - dx); dy = min(dy, n2 - dy); dz = min(dz, n3 - dz);
+                    return max(dx, max(dy, dz)) >= D;
+                };
+                const int ka = sh.sel_ca[k], kp = sh.sel_cp[k], la = sh.sel_ca[l], lp = sh.sel_cp[l];
+                if (!(far(ka, la) && far(ka, lp) && far(kp, la) && far(kp, lp))) mine |= 1ull << k;
             }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) mine |= __shfl_xor(mine, off, 64);
+            if (lane == 0 && mine) atomicOr(&sh.conflict, mine);
+        }
+        __syncthreads();
+        if (wave == 0) {
+            const int ns = sh.nsort;
+            const int a = sh.sel_a[lane];
+            const bool ok = lane < ns && a >= 1 && a <= S.N && !((sh.conflict >> lane) & 1);
             // keep sorted order; cap the batch
             const unsigned long long okm = __ballot(ok);
             const int rank = __popcll(okm & ((1ULL << lane) - 1));
@@ -1126,51 +1492,50 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
             if (lane == 0) {
                 const int nok = __popcll(okm);
                 sh.nex = min(nok, cap);
-                F.batches++;
-                if (single) F.single++;
-                F.deferred += nsel - sh.nex;
-                sh.ndirty = 0;
+                F.deferred += ns - sh.nex;
             }
         }
         __syncthreads();
+        PROF_MARK(S, FPH_SELECT, pt);
         // ---- execution: one wave per event
         const int nex = sh.nex;
         if (nex < 1) {                                     // no selectable event: stop with an error
             if (tid == 0) sh.err = FERR_SELECT;
             break;
         }
+        // events spread over the waves' lanes (event e on wave e % W, lane e / W)
         int err = 0;
-        for (int e = wave; e < nex; e += W) {
-            const int a = sh.ex_a[e];
-            const double t = S.evt[a];
-            int renew[2] = {0, 0};
-            int nr = 0;
-            if (lane == 0) {
-                nr = exec_fast(F, a, t, renew, &err);
+        {
+            const int e = lane * W + wave;
+            if (e < nex) {
+                const int a = sh.ex_a[e];
+                const double t = G(S.evt)[a];
+                int renew[2] = {0, 0};
+                const int nr = exec_fast(F, a, t, renew, &err);
                 put_record(F, renew[0]);
                 if (nr > 1) put_record(F, renew[1]);
                 cnt_max(F.tlast, t);
-            }
-            nr = __shfl(nr, 0, 64);
-            renew[0] = __shfl(renew[0], 0, 64);
-            renew[1] = __shfl(renew[1], 0, 64);
-            err = __shfl(err, 0, 64);
-            __threadfence_block();
-            __builtin_amdgcn_wave_barrier();
-            if (err) break;
-            for (int r = 0; r < nr; r++) {
-                predict_fast(F, sh.w[wave], renew[r], t, ws.screen, ws.conf, ws.fp64, &err);
-                __threadfence_block();
-                __builtin_amdgcn_wave_barrier();
-            }
-            if (lane == 0) {
-                const int d = atomicAdd(&sh.ndirty, nr);
-                for (int r = 0; r < nr && d + r < 4 * FSEL; r++) sh.dirty[d + r] = (renew[r] - 1) >> 6;
+                const int slot0 = atomicAdd(&sh.nren, nr);
+                for (int r = 0; r < nr; r++) { sh.ren_a[slot0 + r] = renew[r]; sh.ren_t[slot0 + r] = t; }
             }
         }
         if (err) atomicCAS(&sh.err, 0, err);
         __syncthreads();
+        PROF_MARK(S, FPH_EVENT, pt);
+        // re-prediction of every renewed atom, one wave per atom
+        const int nren = sh.nren;
+        err = 0;
+        for (int r = wave; r < nren && sh.err == 0; r += W) {
+            const int a = sh.ren_a[r];
+            predict_fast(F, tab, sh.w[wave], a, sh.ren_t[r], ws, &err);
+            if (__ballot(err != 0)) break;          // uniform across the wave
+        }
+        for (int r = tid; r < nren; r += 64 * W) sh.dirty[r] = (sh.ren_a[r] - 1) >> 6;
+        if (tid == 0) sh.ndirty = nren;
+        if (err) atomicCAS(&sh.err, 0, err);
+        __syncthreads();
         if (tid == 0) S.pops += nex;
+        PROF_MARK(S, FPH_EXEC, pt);
         // ---- tree: rescan the touched blocks
         const int nd = min(sh.ndirty, 4 * FSEL);
         for (int d = wave; d < nd; d += W) {
@@ -1178,18 +1543,23 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
             bool dup = false;
             for (int q = 0; q < d; q++) dup |= (sh.dirty[q] == b);
             if (dup || b < 0 || b >= nblk) continue;
-            double t1, t2;
-            int a1b;
-            block_scan(F, b, t1, a1b, t2);
-            if (lane == 0) { sh.b1[b] = t1; sh.ba[b] = a1b; sh.b2[b] = t2; }
+            double t1, t2, t3;
+            int a1b, a2b;
+            block_scan(F, b, t1, a1b, t2, a2b, t3);
+            if (lane == 0) { sh.b1[b] = t1; sh.ba[b] = a1b; sh.b2[b] = t2; sh.bb[b] = a2b; sh.b3[b] = t3; }
         }
         __syncthreads();
+        PROF_MARK(S, FPH_TREE, pt);
     }
-    for (int b = tid; b < nblk; b += 64 * W) { F.b1[b] = sh.b1[b]; F.b2[b] = sh.b2[b]; F.ba[b] = sh.ba[b]; }
+    for (int b = tid; b < nblk; b += 64 * W) {
+        G(F.b1)[b] = sh.b1[b]; G(F.b2)[b] = sh.b2[b]; G(F.b3)[b] = sh.b3[b];
+        G(F.ba)[b] = sh.ba[b]; G(F.bb)[b] = sh.bb[b];
+    }
     if (lane == 0) {
         atomicAdd((unsigned long long *)&F.screens, (unsigned long long)ws.screen);
         atomicAdd((unsigned long long *)&F.confirms, (unsigned long long)ws.conf);
         atomicAdd((unsigned long long *)&S.pair_evals, (unsigned long long)(ws.fp64 + ws.conf));
+        atomicAdd((unsigned long long *)&S.candidates, (unsigned long long)ws.cand);
     }
     __syncthreads();
     if (tid == 0) {
@@ -1237,10 +1607,12 @@ int sdmd_fast_open(sdmd_gpu_ctx *c, const sdmd_gpu_opts *o) {
         F.ncell = (int)ncell;
         F.nblk = (N + 63) / 64;
         if (F.nblk > FMAXBLK || N >= (1 << 23)) return SDMD_EUNSUPPORTED;
+        if (d->cellnum[0] >= 1024 || d->cellnum[1] >= 1024 || d->cellnum[2] >= 1024) return SDMD_EUNSUPPORTED;
         for (int k = 0; k < 3; k++) F.L32[k] = (float)d->cellsize[k];
         F.Te = d->currenttime;
         F.tlast = d->currenttime;
         F.rebase_dt = 0.5;
+        F.t_stop = INFINIT;
         const unsigned long long seed = (o && o->seed) ? o->seed : (unsigned long long)d->seed;
         F.key0 = (unsigned int)seed;
         F.key1 = (unsigned int)(seed >> 32) ^ (0x9E3779B9u * (unsigned int)(s + 1));
@@ -1268,11 +1640,52 @@ int sdmd_fast_open(sdmd_gpu_ctx *c, const sdmd_gpu_opts *o) {
         if (rc == SDMD_OK) rc = fdev_alloc(c, &F.t0, t0, (size_t)N + 1);
         free(cnt); free(mem); free(slot); free(t0);
         if (rc) return rc;
-        if ((rc = fdev_alloc(c, &F.rec, (const float4 *)nullptr, ((size_t)ncell + 1) * FCAP * 2))) return rc;
+        if ((rc = fdev_alloc(c, &F.rec, (const fvec4 *)nullptr, ((size_t)ncell + 1) * FCAP * 2))) return rc;
         if ((rc = fdev_alloc(c, &F.b1, (const double *)nullptr, (size_t)F.nblk))) return rc;
         if ((rc = fdev_alloc(c, &F.b2, (const double *)nullptr, (size_t)F.nblk))) return rc;
         if ((rc = fdev_alloc(c, &F.ba, (const int *)nullptr, (size_t)F.nblk))) return rc;
+        if ((rc = fdev_alloc(c, &F.b3, (const double *)nullptr, (size_t)F.nblk))) return rc;
+        if ((rc = fdev_alloc(c, &F.bb, (const int *)nullptr, (size_t)F.nblk))) return rc;
         if ((rc = fdev_alloc(c, &F.rctr, (const unsigned int *)nullptr, (size_t)N + 1))) return rc;
+        // FP32 screen table: RightPair (ToolFunctions.c:278-292) resolved for
+        // every type pair, the pair's limits and its step radii
+        ScrTab *tb = (ScrTab *)calloc(1, sizeof(ScrTab));
+        if (!tb) return SDMD_ENOMEM;
+        auto back = [](int t) { return t == 3 ? 1 : t == 4 ? 2 : t == 27 ? 26 : t; };
+        int pool = 0;
+        rc = SDMD_OK;
+        for (int t1 = 0; t1 < NT; t1++)
+            for (int t2 = 0; t2 < NT; t2++) {
+                const int a = back(t1), b = back(t2);
+                const int cand[4] = {t1 * NT + t2, a * NT + t2, t1 * NT + b, a * NT + b};
+                short r = -1;
+                for (int q = 0; q < 4 && r < 0; q++)
+                    if (d->pair[cand[q]].dmin > 0) r = (short)cand[q];
+                tb->rp[t1 * NT + t2] = r;
+            }
+        for (int p = 0; p < NT * NT; p++) {
+            const sdmd_pair &pp = d->pair[p];
+            if (!(pp.dmin > 0)) continue;
+            if (pp.nstep > 16 || pool + pp.nstep > FRAD) { rc = SDMD_EUNSUPPORTED; break; }
+            ScrPair &sp = tb->pr[p];
+            sp.dmin = (float)pp.dmin;
+            sp.dmax = (float)pp.dmax;
+            sp.nstep = (unsigned short)pp.nstep;
+            sp.off = (unsigned short)pool;
+            sp.ezero = 0;
+            for (int k = 0; k < pp.nstep; k++) {
+                tb->rad[pool + k] = (float)d->step_d[pp.step0 + k];
+                if (d->step_e[pp.step0 + k] == 0) sp.ezero |= 1u << k;
+            }
+            pool += pp.nstep;
+        }
+        if (rc == SDMD_OK) {
+            ScrTab *dt = nullptr;
+            rc = fdev_alloc(c, &dt, (const ScrTab *)tb, 1);
+            F.scrtab = dt;
+        }
+        free(tb);
+        if (rc) return rc;
     }
     if ((rc = fdev_alloc(c, &fs->dev, fs->host, (size_t)c->nsys))) return rc;
     return SDMD_OK;
@@ -1310,8 +1723,11 @@ int sdmd_fast_prepare(sdmd_gpu_ctx *c) {
     return SDMD_OK;
 }
 
-int sdmd_fast_run(sdmd_gpu_ctx *c, long long pops, double max_ms) {
+int sdmd_fast_run(sdmd_gpu_ctx *c, long long pops, double max_ms, double t_end) {
     FastState *fs = c->fast;
+    for (int s = 0; s < c->nsys; s++) fs->host[s].t_stop = t_end;
+    if (hipMemcpyAsync(fs->dev, fs->host, sizeof(FSys) * c->nsys, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        return SDMD_EDEVICE;
     long long budget = 1LL << 62;
     if (max_ms > 0) {
         int dev = 0, khz = 0;

@@ -133,27 +133,27 @@ __device__ void wave_select(int n, const double *ct, const unsigned char *fl, co
 // JobAssign / CheckDuplication result applied to the staged record (lane 0)
 __device__ __forceinline__ void take_computed(Atom &A, const Sys &S, double t, double pot, int sub, int type, int j) {
     A.pot = pot; A.sub = sub; A.t = t; A.evtype = type;
-    if (j) { A.partner = j; A.pc = S.evc[j]; }
+    if (j) { A.partner = j; A.pc = G(S.evc)[j]; }
     else A.partner = INVALID;
 }
 
 __device__ __forceinline__ void take_inherited(Atom &A, const Sys &S, int j, int type) {
-    A.partner = j; A.pot = S.evpot[j]; A.sub = S.evsub[j]; A.t = S.evt[j]; A.evtype = type; A.pc = S.evc[j];
+    A.partner = j; A.pot = G(S.evpot)[j]; A.sub = G(S.evsub)[j]; A.t = G(S.evt)[j]; A.evtype = type; A.pc = G(S.evc)[j];
 }
 
 __device__ __forceinline__ bool dupable(const Sys &S, int i, long long ci, int j, int type) {
-    return S.evp[j] == i && S.evtype[j] == type && S.evpc[j] == ci;
+    return G(S.evp)[j] == i && G(S.evtype)[j] == type && G(S.evpc)[j] == ci;
 }
 
 // ---------------------------------------------------------------- candidate evaluation
 // one BondTime entry, TimePrediction.c:219-293
 __device__ void eval_bond(const Sys &S, Smem &sm, int q, int *err) {
     const Atom &A = sm.A;
-    const int e = S.bond_off[A.i] + q;
-    const int j = S.bond_j[e];
+    const int e = G(S.bond_off)[A.i] + q;
+    const int j = G(S.bond_j)[e];
     sm.bj[q] = j;
     sm.bfl[q] = dupable(S, A.i, A.counter, j, SDMD_BOND) ? 1 : 0;
-    const double lo = S.bond_dmin[e], hi = S.bond_dmax[e];
+    const double lo = G(S.bond_dmin)[e], hi = G(S.bond_dmax)[e];
     Kin k;
     calc_params(S, A.x, A.y, A.z, A.vx, A.vy, A.vz, A.cx, A.cy, A.cz, j, k);
     double t;
@@ -237,13 +237,13 @@ __device__ void eval_hb(const Sys &S, Smem &sm, int kidx, double calib, int *err
     double tout = __builtin_inf();
     double pot = 0;
     unsigned char fl = 0;
-    const signed char ri = A.role, rj = S.role[j];
+    const signed char ri = A.role, rj = G(S.role)[j];
     const bool pairok = (ri == 'A' && rj == 'D') || (ri == 'D' && rj == 'A') ||
-                        (ri == 'S' && S.type[j] == 2) || (rj == 'S' && A.type == 2) || (ri == 'S' && rj == 'S');
-    int sep = A.aanum - S.aanum[j];
+                        (ri == 'S' && G(S.type)[j] == 2) || (rj == 'S' && A.type == 2) || (ri == 'S' && rj == 'S');
+    int sep = A.aanum - G(S.aanum)[j];
     if (sep < 0) sep = -sep;
-    const int pj = S.protein[j];
-    if (j != A.i && pairok && S.hbbond[j] == 0 && ((A.protein == pj && sep >= 4) || A.protein != pj)) {
+    const int pj = G(S.protein)[j];
+    if (j != A.i && pairok && G(S.hbbond)[j] == 0 && ((A.protein == pj && sep >= 4) || A.protein != pj)) {
         if (dupable(S, A.i, A.counter, j, SDMD_HB)) fl = 1;
         Kin k;
         calc_params(S, A.x, A.y, A.z, A.vx, A.vy, A.vz, A.cx, A.cy, A.cz, j, k);
@@ -274,7 +274,7 @@ __device__ void job_assign_atom(Atom &A, const Sys &S, double t, double pot, int
 
 __device__ double next_normal(Sys &S, Smem &sm) {
     if (sm.zpos >= S.nz) { sm.err = ERR_NORMALS; return 1.0; }
-    return S.zn[sm.zpos++];
+    return G(S.zn)[sm.zpos++];
 }
 
 // PBCandCrossCellTime, TimePrediction.c:598-632
@@ -303,7 +303,7 @@ __device__ void pcc_time(const Sys &S, Atom &A) {
 // HBTime for a bonded HB atom, TimePrediction.c:330-383
 __device__ void hb_time_bonded(const Sys &S, Atom &A, int *err) {
     const int j = A.hbbond;
-    if (dupable(S, A.i, A.counter, j, SDMD_HB) && A.t > S.evt[j]) { take_inherited(A, S, j, SDMD_HB); return; }
+    if (dupable(S, A.i, A.counter, j, SDMD_HB) && A.t > G(S.evt)[j]) { take_inherited(A, S, j, SDMD_HB); return; }
     Kin k;
     calc_params(S, A.x, A.y, A.z, A.vx, A.vy, A.vz, A.cx, A.cy, A.cz, j, k);
     FP f;
@@ -328,7 +328,7 @@ __device__ void hb_neighbor_time(const Sys &S, Atom &A, int *err) {
     for (int n = 1; n <= A.nbstat; n++) {
         const int j = A.nbp[n];
         if (flag == 0) A.nbnum = n;
-        if (dupable(S, A.i, A.counter, j, SDMD_HBNE) && A.t > S.evt[j]) {
+        if (dupable(S, A.i, A.counter, j, SDMD_HBNE) && A.t > G(S.evt)[j]) {
             take_inherited(A, S, j, SDMD_HBNE);
             A.nbnum = n;
             flag = 1;
@@ -338,8 +338,8 @@ __device__ void hb_neighbor_time(const Sys &S, Atom &A, int *err) {
         calc_params(S, A.x, A.y, A.z, A.vx, A.vy, A.vz, A.cx, A.cy, A.cz, j, k);
         int host, nb, partner;
         const bool ads = (A.role == 'A' || A.role == 'D' || A.role == 'S');
-        if (ads && A.hbbond && S.hbnb[A.hbbond] == j) { host = A.i; nb = j; partner = A.hbbond; }
-        else { host = j; nb = A.i; partner = S.hbbond[j]; }
+        if (ads && A.hbbond && G(S.hbnb)[A.hbbond] == j) { host = A.i; nb = j; partner = A.hbbond; }
+        else { host = j; nb = A.i; partner = G(S.hbbond)[j]; }
         FP f;
         int cs = find_pair(S, nb, host, K_NEIGHBOR, k.b, k.rr, f, partner, 0, err);
         int s, sub;
@@ -365,26 +365,26 @@ __device__ void hb_neighbor_time(const Sys &S, Atom &A, int *err) {
 __device__ void stage_atom(Sys &S, Smem &sm, int i) {
     Atom &A = sm.A;
     A.i = i;
-    A.x = S.x[i]; A.y = S.y[i]; A.z = S.z[i];
-    A.vx = S.vx[i]; A.vy = S.vy[i]; A.vz = S.vz[i];
-    A.counter = S.evc[i];
-    A.cx = S.cx[i]; A.cy = S.cy[i]; A.cz = S.cz[i];
-    A.type = S.type[i]; A.hbbond = S.hbbond[i]; A.hbnb = S.hbnb[i];
-    A.protein = S.protein[i]; A.aanum = S.aanum[i]; A.role = S.role[i];
-    A.nbstat = S.nbstat[i]; A.nbnum = S.nbnum[i];
-    for (int k = 0; k < SDMD_NBSLOTS; k++) A.nbp[k] = S.nbp[(size_t)i * SDMD_NBSLOTS + k];
-    A.t = S.evt[i]; A.pot = S.evpot[i]; A.pc = S.evpc[i];
-    A.partner = S.evp[i]; A.evtype = S.evtype[i]; A.sub = S.evsub[i];
-    const int o = S.cmap_off[i], n = S.cmap_off[i + 1] - o;
+    A.x = G(S.x)[i]; A.y = G(S.y)[i]; A.z = G(S.z)[i];
+    A.vx = G(S.vx)[i]; A.vy = G(S.vy)[i]; A.vz = G(S.vz)[i];
+    A.counter = G(S.evc)[i];
+    A.cx = G(S.cx)[i]; A.cy = G(S.cy)[i]; A.cz = G(S.cz)[i];
+    A.type = G(S.type)[i]; A.hbbond = G(S.hbbond)[i]; A.hbnb = G(S.hbnb)[i];
+    A.protein = G(S.protein)[i]; A.aanum = G(S.aanum)[i]; A.role = G(S.role)[i];
+    A.nbstat = G(S.nbstat)[i]; A.nbnum = G(S.nbnum)[i];
+    for (int k = 0; k < SDMD_NBSLOTS; k++) A.nbp[k] = G(S.nbp)[(size_t)i * SDMD_NBSLOTS + k];
+    A.t = G(S.evt)[i]; A.pot = G(S.evpot)[i]; A.pc = G(S.evpc)[i];
+    A.partner = G(S.evp)[i]; A.evtype = G(S.evtype)[i]; A.sub = G(S.evsub)[i];
+    const int o = G(S.cmap_off)[i], n = G(S.cmap_off)[i + 1] - o;
     A.nrow = n;
-    sm.nb = S.bond_off[i + 1] - S.bond_off[i];
+    sm.nb = G(S.bond_off)[i + 1] - G(S.bond_off)[i];
 }
 
 __device__ void unstage_atom(Sys &S, const Atom &A) {
     const int i = A.i;
-    S.evt[i] = A.t; S.evpot[i] = A.pot; S.evpc[i] = A.pc;
-    S.evp[i] = A.partner; S.evtype[i] = A.evtype; S.evsub[i] = A.sub;
-    S.nbnum[i] = A.nbnum;
+    G(S.evt)[i] = A.t; G(S.evpot)[i] = A.pot; G(S.evpc)[i] = A.pc;
+    G(S.evp)[i] = A.partner; G(S.evtype)[i] = A.evtype; G(S.evsub)[i] = A.sub;
+    G(S.nbnum)[i] = A.nbnum;
 }
 
 // Predict for one atom whose record was already reset (ResetTarget).
@@ -403,13 +403,13 @@ __device__ void predict_atom(Sys &S, Smem &sm, int i) {
     const int nrow = sm.A.nrow;
     if (nrow <= ROWCAP)
         for (int k = tid; k < nrow; k += B) {
-            sm.row_j[k] = S.cmap_j[S.cmap_off[i] + k];
-            sm.row_b[k] = S.cmap_bits[S.cmap_off[i] + k];
+            sm.row_j[k] = G(S.cmap_j)[G(S.cmap_off)[i] + k];
+            sm.row_b[k] = G(S.cmap_bits)[G(S.cmap_off)[i] + k];
         }
     if (tid >= B - SDMD_DYNSLOTS) {
         const int k = tid - (B - SDMD_DYNSLOTS);
-        sm.dyn_j[k] = S.dynj[(size_t)i * SDMD_DYNSLOTS + k];
-        sm.dyn_b[k] = S.dynb[(size_t)i * SDMD_DYNSLOTS + k];
+        sm.dyn_j[k] = G(S.dynj)[(size_t)i * SDMD_DYNSLOTS + k];
+        sm.dyn_b[k] = G(S.dynb)[(size_t)i * SDMD_DYNSLOTS + k];
     }
     int err = 0;
     const int nb = sm.nb;
@@ -421,12 +421,12 @@ __device__ void predict_atom(Sys &S, Smem &sm, int i) {
     const int glane = (B >= 128) ? lane : lane - 32;
     if (wave == gwave && glane >= 0 && glane < 27) {
         const int c = glane;
-        int j = S.celllist[S.N + cell_index(S, sm.A.cx, sm.A.cy, sm.A.cz, c)];
+        int j = G(S.celllist)[S.N + cell_index(S, sm.A.cx, sm.A.cy, sm.A.cz, c)];
         int cnt = 0;
         while (j && cnt <= CELL_CAP) {      // bounded walk: an over-full cell is an error
             if (cnt < CELL_CAP) sm.cellbuf[c * CELL_CAP + cnt] = j;
             cnt++;
-            j = S.celllist[j];
+            j = G(S.celllist)[j];
         }
         sm.cellcnt[c] = cnt;
     }
@@ -494,10 +494,10 @@ __device__ void predict_atom(Sys &S, Smem &sm, int i) {
             // HBTime head, TimePrediction.c:309-330
             sm.hbscan = 0;
             const int i0 = sm.A.i;
-            if (S.hbitype[i0] == SDMD_HBSKIP) {
-                S.hbitype[i0] = SDMD_NOEVENT;
+            if (G(S.hbitype)[i0] == SDMD_HBSKIP) {
+                G(S.hbitype)[i0] = SDMD_NOEVENT;
             } else {
-                S.hbitype[i0] = SDMD_NOEVENT;
+                G(S.hbitype)[i0] = SDMD_NOEVENT;
                 if (sm.A.hbbond > 0) hb_time_bonded(S, sm.A, &err);
                 else if (sm.A.role == 'A' || sm.A.role == 'D' || sm.A.role == 'S') sm.hbscan = 1;
             }
@@ -545,9 +545,9 @@ __device__ void predict_list(Sys &S, Smem &sm, const int *renew) {
     if (threadIdx.x == 0) {
         for (int n = 1; renew[n]; n++) {
             const int i = renew[n];
-            S.evtype[i] = SDMD_INVD;
-            S.evt[i] = INFINIT;
-            S.evp[i] = INVALID;
+            G(S.evtype)[i] = SDMD_INVD;
+            G(S.evt)[i] = INFINIT;
+            G(S.evp)[i] = INVALID;
         }
     }
     __syncthreads();
@@ -557,32 +557,32 @@ __device__ void predict_list(Sys &S, Smem &sm, const int *renew) {
 // ---------------------------------------------------------------- CBT (lane 0), CBT.c
 __device__ void cbt_update_run(Sys &S, int a) {
     int father, left, right, old;
-    for (father = S.leaf[a] / 2; father > 0; father /= 2) {
-        if (S.node[father] != a) break;
-        left = S.node[father * 2];
-        right = S.node[father * 2 + 1];
-        S.node[father] = (S.evt[left] < S.evt[right]) ? left : right;
+    for (father = G(S.leaf)[a] / 2; father > 0; father /= 2) {
+        if (G(S.node)[father] != a) break;
+        left = G(S.node)[father * 2];
+        right = G(S.node)[father * 2 + 1];
+        G(S.node)[father] = (G(S.evt)[left] < G(S.evt)[right]) ? left : right;
     }
     for (; father > 0; father /= 2) {
-        old = S.node[father];
-        left = S.node[father * 2];
-        right = S.node[father * 2 + 1];
-        S.node[father] = (S.evt[left] < S.evt[right]) ? left : right;
-        if (S.node[father] == old) return;
+        old = G(S.node)[father];
+        left = G(S.node)[father * 2];
+        right = G(S.node)[father * 2 + 1];
+        G(S.node)[father] = (G(S.evt)[left] < G(S.evt)[right]) ? left : right;
+        if (G(S.node)[father] == old) return;
     }
 }
 
 __device__ void cbt_create(Sys &S) {
-    for (int k = 0; k < 2 * S.N; k++) S.node[k] = 0;
-    for (int k = 0; k <= S.N; k++) S.leaf[k] = 0;
+    for (int k = 0; k < 2 * S.N; k++) G(S.node)[k] = 0;
+    for (int k = 0; k <= S.N; k++) G(S.leaf)[k] = 0;
     S.count = 0;
     for (int a = 1; a <= S.N; a++) {
-        if (S.count == 0) { S.node[1] = a; S.count++; continue; }
-        const int i = S.node[S.count];
-        S.node[S.count * 2] = i;
-        S.node[S.count * 2 + 1] = a;
-        S.leaf[i] = S.count * 2;
-        S.leaf[a] = S.count * 2 + 1;
+        if (S.count == 0) { G(S.node)[1] = a; S.count++; continue; }
+        const int i = G(S.node)[S.count];
+        G(S.node)[S.count * 2] = i;
+        G(S.node)[S.count * 2 + 1] = a;
+        G(S.leaf)[i] = S.count * 2;
+        G(S.leaf)[a] = S.count * 2 + 1;
         S.count++;
         cbt_update_run(S, i);
     }
@@ -593,17 +593,17 @@ __device__ void thermostat_event(Sys &S, Smem &sm, int i) {
     double sp[3];
     for (int n = 0; n < 3; n++) {
         do {
-            const double variance = S.T / S.mass[i];
+            const double variance = S.T / G(S.mass)[i];
             sp[n] = next_normal(S, sm) * sqrt(variance) + 0;
             sp[n] += 0.0 + 0.0 + 0.0 * (S.currenttime - 0.0);
         } while (sp[n] == 0 && sm.err == 0);
     }
-    S.vx[i] = sp[0]; S.vy[i] = sp[1]; S.vz[i] = sp[2];
+    G(S.vx)[i] = sp[0]; G(S.vy)[i] = sp[1]; G(S.vz)[i] = sp[2];
 }
 
 // PBCandCrossCellEvent + PBC("part") + LinkList("part"), Events.c:158-272
 __device__ void pcc_event(Sys &S, int i, int *err) {
-    const int old[3] = {S.cx[i], S.cy[i], S.cz[i]};
+    const int old[3] = {G(S.cx)[i], G(S.cy)[i], G(S.cz)[i]};
     double *c[3] = {S.x, S.y, S.z};
     int *ci[3] = {S.cx, S.cy, S.cz};
     for (int n = 0; n < 3; n++) {
@@ -623,26 +623,26 @@ __device__ void pcc_event(Sys &S, int i, int *err) {
         if (v < 0 || v >= S.cellnum[n + 1]) { *err = ERR_CELL; return; }
         ci[n][i] = v;
     }
-    const int newc = S.cz[i] * S.cellnum[1] * S.cellnum[2] + S.cy[i] * S.cellnum[1] + S.cx[i] + 1;
+    const int newc = G(S.cz)[i] * S.cellnum[1] * S.cellnum[2] + G(S.cy)[i] * S.cellnum[1] + G(S.cx)[i] + 1;
     int nx = oldc + N, guard = 0;
-    while (S.celllist[nx] != i) {
-        nx = S.celllist[nx];
+    while (G(S.celllist)[nx] != i) {
+        nx = G(S.celllist)[nx];
         if (nx == 0 || ++guard > N) { *err = ERR_LINKLIST; return; }
     }
-    S.celllist[nx] = S.celllist[i];
+    G(S.celllist)[nx] = G(S.celllist)[i];
     nx = newc + N;
-    S.celllist[i] = S.celllist[nx];
-    S.celllist[nx] = i;
+    G(S.celllist)[i] = G(S.celllist)[nx];
+    G(S.celllist)[nx] = i;
 }
 
 __device__ void do_event(Sys &S, Smem &sm, int t, int p, int *err) {   // DoEvent, Events.c:25-114
-    const int ty = S.evtype[t];
+    const int ty = G(S.evtype)[t];
     switch (ty) {
     case SDMD_COLI: case SDMD_BOND: case SDMD_HBNE:
         interaction_event(S, PosStored{S}, t, p, 0, 0, err);
         break;
     case SDMD_HB: {
-        const int nbi = S.hbnb[t], nbj = S.hbnb[p];
+        const int nbi = G(S.hbnb)[t], nbj = G(S.hbnb)[p];
         interaction_event(S, PosStored{S}, t, p, nbi, nbj, err);
         hb_event(S, t, p, nbi, nbj, err);
         break;
@@ -663,8 +663,8 @@ __device__ unsigned long long fnv(unsigned long long h, const void *p, int n) {
 __device__ unsigned long long state_hash(const Sys &S) {
     unsigned long long h = 1469598103934665603ULL;
     for (int i = 1; i <= S.N; i++) {
-        const double p[3] = {S.x[i], S.y[i], S.z[i]}, v[3] = {S.vx[i], S.vy[i], S.vz[i]};
-        const long long c = S.evc[i];
+        const double p[3] = {G(S.x)[i], G(S.y)[i], G(S.z)[i]}, v[3] = {G(S.vx)[i], G(S.vy)[i], G(S.vz)[i]};
+        const long long c = G(S.evc)[i];
         h = fnv(h, p, sizeof p);
         h = fnv(h, v, sizeof v);
         h = fnv(h, &S.evt[i], sizeof(double));
@@ -724,9 +724,9 @@ __global__ __launch_bounds__(B) void k_prepare(Sys *systems) {
     __syncthreads();
     // ResetTarget for 1..N, then Predict(1..N) in order
     for (int i = tid + 1; i <= S.N; i += B) {
-        S.evtype[i] = SDMD_INVD;
-        S.evt[i] = INFINIT;
-        S.evp[i] = INVALID;
+        G(S.evtype)[i] = SDMD_INVD;
+        G(S.evt)[i] = INFINIT;
+        G(S.evp)[i] = INVALID;
     }
     __syncthreads();
     for (int i = 1; i <= S.N && sm.err == 0; i++) predict_atom<B>(S, sm, i);
@@ -739,7 +739,7 @@ __global__ __launch_bounds__(B) void k_prepare(Sys *systems) {
 }
 
 template <int B>
-__global__ __launch_bounds__(B) void k_run(Sys *systems, long long npops, long long budget) {
+__global__ __launch_bounds__(B) void k_run(Sys *systems, long long npops, long long budget, double t_end) {
     __shared__ Smem sm;
     __shared__ Sys Sl;
     __shared__ int stop;
@@ -755,19 +755,19 @@ __global__ __launch_bounds__(B) void k_run(Sys *systems, long long npops, long l
         if (sm.err) break;
         if (tid == 0) {
             // device-time budget of a bounded run
-            if ((long long)(wall_clock64() - t_start) > budget) stop = 1;
+            if ((long long)(wall_clock64() - t_start) > budget || S.currenttime > t_end) stop = 1;
             // AssignJob, SGThread.c:237-259
-            const int target = S.node[1];
+            const int target = G(S.node)[1];
             sm.target = target;
-            sm.dt = S.evt[target];
+            sm.dt = G(S.evt)[target];
             sm.renew[0] = 1; sm.renew[1] = target; sm.renew[2] = 0; sm.renew[3] = 0;
             sm.partner = 0;
-            if (S.evp[target] > 0) { sm.partner = S.evp[target]; sm.renew[++sm.renew[0]] = sm.partner; }
+            if (G(S.evp)[target] > 0) { sm.partner = G(S.evp)[target]; sm.renew[++sm.renew[0]] = sm.partner; }
             if (S.trace) {
                 if (S.trace_n < S.trace_cap) {
                     sdmd_gpu_pop_rec &r = S.trace[S.trace_n];
-                    r.target = target; r.partner = S.evp[target]; r.type = S.evtype[target];
-                    r.sub = S.evsub[target]; r.dt = sm.dt; r.pot = S.evpot[target];
+                    r.target = target; r.partner = G(S.evp)[target]; r.type = G(S.evtype)[target];
+                    r.sub = G(S.evsub)[target]; r.dt = sm.dt; r.pot = G(S.evpot)[target];
                 } else {
                     sm.err = ERR_TRACE;
                 }
@@ -784,13 +784,13 @@ __global__ __launch_bounds__(B) void k_run(Sys *systems, long long npops, long l
             S.currenttime += sm.dt;
             const int target = sm.target, partner = sm.partner;
             // HazardCheck, SGThread.c:215-235
-            sm.stale = (partner > 0 && S.evpc[target] < S.evc[partner]);
+            sm.stale = (partner > 0 && G(S.evpc)[target] < G(S.evc)[partner]);
             if (!sm.stale) {
                 int err = 0;
                 S.frame++;
                 do_event(S, sm, target, partner, &err);
-                S.evc[target]++;
-                if (sm.renew[0] == 2) S.evc[partner]++;
+                G(S.evc)[target]++;
+                if (sm.renew[0] == 2) G(S.evc)[partner]++;
                 if (err) sm.err = err;
             } else {
                 sm.renew[0] = 1; sm.renew[2] = 0; sm.renew[3] = 0;
@@ -813,16 +813,16 @@ __global__ __launch_bounds__(B) void k_run(Sys *systems, long long npops, long l
                 int n = 0;
                 for (int k = 1; sm.renew[k] && n < 2; k++, n++) {
                     const int a = sm.renew[k];
-                    r.r[n].atom = a; r.r[n].partner = S.evp[a]; r.r[n].type = S.evtype[a];
-                    r.r[n].sub = S.evsub[a]; r.r[n].counter = S.evc[a]; r.r[n].pcounter = S.evpc[a];
-                    r.r[n].time = S.evt[a]; r.r[n].pot = S.evpot[a];
+                    r.r[n].atom = a; r.r[n].partner = G(S.evp)[a]; r.r[n].type = G(S.evtype)[a];
+                    r.r[n].sub = G(S.evsub)[a]; r.r[n].counter = G(S.evc)[a]; r.r[n].pcounter = G(S.evpc)[a];
+                    r.r[n].time = G(S.evt)[a]; r.r[n].pot = G(S.evpot)[a];
                 }
                 r.nrenew = n;
                 S.trace_n++;
             }
             if (S.hash_every > 0 && S.pops % S.hash_every == 0 && S.hash_n < S.hash_cap) {
-                S.hashes[2 * S.hash_n] = S.pops;
-                S.hashes[2 * S.hash_n + 1] = (long long)state_hash(S);
+                G(S.hashes)[2 * S.hash_n] = S.pops;
+                G(S.hashes)[2 * S.hash_n + 1] = (long long)state_hash(S);
                 S.hash_n++;
             }
         }
@@ -851,10 +851,10 @@ __global__ __launch_bounds__(B) void k_op(Sys *systems, int sys, int op, const i
         predict_list<B>(S, sm, renew);
     } else if (op == OP_ADVANCE) {
         for (int i = tid + 1; i <= S.N; i += B) {
-            S.x[i] += S.vx[i] * dt;
-            S.y[i] += S.vy[i] * dt;
-            S.z[i] += S.vz[i] * dt;
-            S.evt[i] -= dt;
+            G(S.x)[i] += G(S.vx)[i] * dt;
+            G(S.y)[i] += G(S.vy)[i] * dt;
+            G(S.z)[i] += G(S.vz)[i] * dt;
+            G(S.evt)[i] -= dt;
         }
         __syncthreads();
         if (tid == 0) S.currenttime += dt;
@@ -866,8 +866,8 @@ __global__ __launch_bounds__(B) void k_op(Sys *systems, int sys, int op, const i
             // SingleThreadRun, SGThread.c:100-104
             S.frame++;
             do_event(S, sm, a, b, &err);
-            S.evc[a]++;
-            if (b > 0) S.evc[b]++;
+            G(S.evc)[a]++;
+            if (b > 0) G(S.evc)[b]++;
         } else if (op == OP_CELL_MOVE) pcc_event(S, a, &err);
         if (err) sm.err = err;
     }
@@ -1041,8 +1041,8 @@ static void launch_prepare(sdmd_gpu_ctx *c) {
     hipLaunchKernelGGL((k_prepare<B>), dim3(c->nsys), dim3(B), 0, c->stream, c->dev_sys);
 }
 template <int B>
-static void launch_run(sdmd_gpu_ctx *c, long long n, long long budget) {
-    hipLaunchKernelGGL((k_run<B>), dim3(c->nsys), dim3(B), 0, c->stream, c->dev_sys, n, budget);
+static void launch_run(sdmd_gpu_ctx *c, long long n, long long budget, double t_end) {
+    hipLaunchKernelGGL((k_run<B>), dim3(c->nsys), dim3(B), 0, c->stream, c->dev_sys, n, budget, t_end);
 }
 template <int B>
 static void launch_op(sdmd_gpu_ctx *c, int s, int op, double dt, int a, int b) {
@@ -1076,11 +1076,21 @@ extern "C" int sdmd_gpu_prepare(sdmd_gpu_ctx *c) {
     return check_errors(c);
 }
 
-extern "C" int sdmd_gpu_run(sdmd_gpu_ctx *c, int64_t pops) { return sdmd_gpu_run_for(c, pops, 0); }
+static int run_bounded(sdmd_gpu_ctx *c, int64_t pops, double max_ms, double t_end);
+
+extern "C" int sdmd_gpu_run(sdmd_gpu_ctx *c, int64_t pops) { return run_bounded(c, pops, 0, INFINIT); }
 
 extern "C" int sdmd_gpu_run_for(sdmd_gpu_ctx *c, int64_t pops, double max_ms) {
+    return run_bounded(c, pops, max_ms, INFINIT);
+}
+
+extern "C" int sdmd_gpu_run_until(sdmd_gpu_ctx *c, double t_end, int64_t max_pops) {
+    return run_bounded(c, max_pops, 0, t_end);
+}
+
+static int run_bounded(sdmd_gpu_ctx *c, int64_t pops, double max_ms, double t_end) {
     if (!c || pops < 0) return SDMD_EINVAL;
-    if (c->mode == SDMD_MODE_FAST) return sdmd_fast_run(c, pops, max_ms);
+    if (c->mode == SDMD_MODE_FAST) return sdmd_fast_run(c, pops, max_ms, t_end);
     long long budget = 1LL << 62;
     if (max_ms > 0) {
         int dev = 0, khz = 0;
@@ -1090,7 +1100,7 @@ extern "C" int sdmd_gpu_run_for(sdmd_gpu_ctx *c, int64_t pops, double max_ms) {
         budget = (long long)(max_ms * khz);
     }
     HIPV(hipEventRecord(c->ev0, c->stream));
-    DISPATCH(launch_run, c, (long long)pops, budget);
+    DISPATCH(launch_run, c, (long long)pops, budget, t_end);
     HIPV(hipEventRecord(c->ev1, c->stream));
     if (hipGetLastError() != hipSuccess) return SDMD_EDEVICE;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return SDMD_EDEVICE;
@@ -1244,7 +1254,6 @@ extern "C" int sdmd_gpu_get_hashes(sdmd_gpu_ctx *c, int32_t s, int64_t *out, int
 // per-phase device time (seconds) of system s when opened with profiling on
 extern "C" int sdmd_gpu_phase_times(sdmd_gpu_ctx *c, int32_t s, double *out, int32_t n) {
     if (!c || s < 0 || s >= c->nsys || !out) return SDMD_EINVAL;
-    if (c->mode != SDMD_MODE_PLAIN) return SDMD_EUNSUPPORTED;
     const Sys &S = c->host_sys[s];
     if (!S.prof) return SDMD_ESTATE;
     unsigned long long v[PH_NPH];

@@ -106,6 +106,7 @@ class GpuEngine:
                   every collision candidate in FP64 and counts agreement."""
 
     PHASES = ["pop", "sweep", "event", "stage", "gather", "bondsel", "eval", "select", "hb", "tail", "tree"]
+    FAST_PHASES = ["select", "exec", "stage", "screen", "confirm", "other", "tree", "event", "bond", "scan", "collect"]
 
     def __init__(self, systems, block=256, trace=False, trace_capacity=0, hash_every=0,
                  normals_per_pop=5.0, max_pops=0, profile=False, mode="plain", validate=False,
@@ -146,15 +147,20 @@ class GpuEngine:
         """Run until every system did max_pops pops or max_ms of device time passed."""
         check(_lib.lib().sdmd_gpu_run_for(self._h, int(max_pops), float(max_ms)), "sdmd_gpu_run_for")
 
+    def run_until(self, t_end, max_pops=1 << 40):
+        """Run until the clock passes t_end (at most max_pops pops per system)."""
+        check(_lib.lib().sdmd_gpu_run_until(self._h, float(t_end), int(max_pops)), "sdmd_gpu_run_until")
+
     def fast_stats(self, sys=0):
         st = FastStats()
         check(_lib.lib().sdmd_gpu_fast_stats_get(self._h, sys, C.byref(st)), "sdmd_gpu_fast_stats_get")
         return {f: getattr(st, f) for f, _ in FastStats._fields_}
 
     def phase_times(self, sys=0):
-        out = np.zeros(len(self.PHASES))
+        names = self.FAST_PHASES if self.mode == "fast" else self.PHASES
+        out = np.zeros(len(names))
         check(_lib.lib().sdmd_gpu_phase_times(self._h, sys, out.ctypes.data, len(out)), "sdmd_gpu_phase_times")
-        return dict(zip(self.PHASES, out))
+        return dict(zip(names, out))
 
     def last_kernel_ms(self):
         ms = C.c_double()

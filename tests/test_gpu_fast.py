@@ -1,0 +1,191 @@
+"""Fast mode (local-time state, FP32 screen + FP64 confirm, concurrent
+independent events) against the CPU oracle and against itself.
+
+The fast engine follows its own trajectory (per-atom local time, its own
+thermostat stream), so the checks are the ones that trajectory allows:
+
+  * bulk prediction on the oracle's start state: the same partner, event
+    type and subtype for every atom and the same time to 1e-9 relative
+    (NVE systems, where no random draw enters the prediction; the slack
+    covers the oracle's inherited times, solved from the mirrored pair);
+  * every screened prediction re-solved against a full FP64 scan
+    (validate mode): 100 % agreement;
+  * schedule invariance: running to the same time with one event per round
+    and with up to 64 concurrent events gives bit-identical states (the
+    events executed together commute);
+  * batching: systems launched together evolve exactly as each alone;
+  * cell arrays: every atom sits in the cell its position indexes;
+  * NVE energy drift of the same order as the reference's own.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from conftest import golden_system
+from sdmd_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "y", "z", "vx", "vy", "vz", "ev_partner", "ev_type", "ev_sub", "ev_time", "cellx", "celly", "cellz")
+
+
+def energy(arr):
+    d = arr.desc()
+    ke, pe = C.c_double(), C.c_double()
+    assert _lib.lib().sdmd_host_energy(C.byref(d), C.byref(ke), C.byref(pe)) == 0
+    return ke.value + pe.value
+
+
+@pytest.fixture(scope="module")
+def mid_nve(tmp_path_factory):
+    from sdmd_amd import builder
+    from sdmd_amd.engine import System
+
+    d = tmp_path_factory.mktemp("midnve")
+    builder.write_input_dir(builder.random_peptides(12, 30, seed=11, mM=5.0), str(d), cutoff=7.25,
+                            thermostat="no")
+    return System.load(str(d))
+
+
+@pytest.fixture(scope="module")
+def mid_nvt(tmp_path_factory):
+    from sdmd_amd import builder
+    from sdmd_amd.engine import System
+
+    d = tmp_path_factory.mktemp("midnvt")
+    builder.write_input_dir(builder.random_peptides(12, 30, seed=11, mM=5.0), str(d), cutoff=7.25)
+    return System.load(str(d))
+
+
+def _bulk_check(system):
+    from sdmd_amd.engine import GpuEngine
+
+    eng = GpuEngine([system], block=256, mode="fast")
+    eng.prepare()
+    st = eng.state(0)
+    eng.close()
+    o = oracle_lib.Oracle(system.arrays)
+    o.bulk()
+    ref = o.state().arr
+    a = st.arr
+    T = st.scalars["currenttime"]
+    for f in ("ev_partner", "ev_type", "ev_sub"):
+        bad = np.flatnonzero(a[f][1:] != ref[f][1:])
+        assert len(bad) == 0, (f, int(bad[0]) + 1)
+    # the oracle copies a partner's time when the partner's event names the
+    # atom (CheckDuplication, TimePrediction.c:1101-1117): that time was
+    # solved from the mirrored pair, whose naive quadratic rounds differently
+    rel = np.abs(a["ev_time"][1:] + T - ref["ev_time"][1:]) / ref["ev_time"][1:]
+    assert rel.max() < 1e-9
+
+
+def test_fast_bulk_prediction_matches_oracle_ala10(require_gpu):
+    s, _ = golden_system("ala10nve")
+    _bulk_check(s)
+
+
+def test_fast_bulk_prediction_matches_oracle_multichain(require_gpu, mid_nve):
+    _bulk_check(mid_nve)
+
+
+def test_fast_screen_agrees_with_full_fp64_scan(require_gpu, mid_nvt):
+    from sdmd_amd.engine import GpuEngine
+
+    eng = GpuEngine([mid_nvt], block=512, mode="fast", validate=True)
+    eng.prepare()
+    eng.run(30000)
+    fs = eng.fast_stats(0)
+    st = eng.stats(0)
+    assert st["error"] == 0
+    assert fs["validated"] > 30000
+    assert fs["agree"] == fs["validated"], fs
+    # the FP32 time of a winner can be far off when the pair is about to
+    # touch (r^2 - d^2 cancels); its error bound sends it to FP64
+    assert fs["max_screen_rel_err"] < 1.0
+    assert fs["screens"] > 10 * fs["confirms"]
+
+
+def test_fast_schedule_invariance(require_gpu, mid_nvt):
+    """One event per round and up to 64 concurrent events reach the same state."""
+    from sdmd_amd.engine import GpuEngine
+
+    states = []
+    for mb in (1, 0):
+        eng = GpuEngine([mid_nvt], block=256, mode="fast", max_batch=mb)
+        eng.prepare()
+        eng.run_until(0.5)
+        states.append((eng.state(0), eng.stats(0), eng.fast_stats(0)))
+        eng.close()
+    (a, sa, fa), (b, sb, fb) = states
+    assert fa["causal"] == 0 and fb["causal"] == 0
+    assert sa["frame"] == sb["frame"] and sa["frame"] > 1000
+    assert fb["batches"] < sb["pops"]            # events did run concurrently
+    for f in FIELDS:
+        assert np.array_equal(a.arr[f], b.arr[f]), f
+
+
+def test_fast_batch_of_systems_is_independent(require_gpu, mid_nvt):
+    from sdmd_amd.engine import GpuEngine
+
+    s2, _ = golden_system("ala10ext")
+    both = GpuEngine([mid_nvt, s2], block=256, mode="fast")
+    both.prepare()
+    both.run_until(0.3)
+    alone = GpuEngine([mid_nvt], block=256, mode="fast")
+    alone.prepare()
+    alone.run_until(0.3)
+    a, b = both.state(0), alone.state(0)
+    for f in FIELDS:
+        assert np.array_equal(a.arr[f], b.arr[f]), f
+
+
+def test_fast_cells_track_positions(require_gpu, mid_nvt):
+    from sdmd_amd.engine import GpuEngine
+
+    eng = GpuEngine([mid_nvt], block=1024, mode="fast")
+    eng.prepare()
+    eng.run(20000)
+    st = eng.state(0)
+    a = st.arr
+    sc = mid_nvt.arrays.scalars
+    cs = np.array(sc["cellsize"][:3])
+    N = mid_nvt.natom
+    pos = np.stack([a["x"][1:], a["y"][1:], a["z"][1:]], 1)
+    cell = np.stack([a["cellx"][1:], a["celly"][1:], a["cellz"][1:]], 1)
+    lo = cell * cs
+    assert np.all(pos >= lo - 1e-6) and np.all(pos < lo + cs + 1e-6)
+    # linked lists rebuilt from the cell arrays hold every atom once
+    seen = np.zeros(N + 1, int)
+    cl = a["celllist"]
+    for c in range(1, len(cl) - N):
+        j = cl[N + c]
+        while j:
+            seen[j] += 1
+            j = cl[j]
+    assert np.all(seen[1:] == 1)
+
+
+def test_fast_nve_energy_drift_within_reference(require_gpu):
+    from sdmd_amd.engine import GpuEngine
+
+    s, _ = golden_system("ala10nve")
+    eng = GpuEngine([s], block=256, mode="fast")
+    eng.prepare()
+    e0 = energy(eng.state(0))
+    fast = []
+    for _ in range(10):
+        eng.run(5000)
+        fast.append(energy(eng.state(0)) - e0)
+    o = oracle_lib.Oracle(s.arrays)
+    o.bulk()
+    ref = []
+    for _ in range(10):
+        o.run(5000, trace=False)
+        ref.append(energy(o.state()) - e0)
+    # the reference's own NVE runs move the total energy in discrete
+    # H-bond bookkeeping steps; the fast trajectory must stay within the
+    # same envelope
+    bound = 2 * max(abs(x) for x in ref) + 1.0
+    assert max(abs(x) for x in fast) <= bound, (fast, ref)
