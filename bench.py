@@ -1,46 +1,75 @@
 """Benchmark: aggregate events/s and pair-time evaluations/s of the sDMD hot
-path on one MI355X.
+path on one MI355X, batched FP32-screen engine against the plain FP64 engine.
 
-Workload (N=1): 64 chains of 40 residues, each a uniform-random sequence over
-the 20 amino acids (seeded), all-atom Ding force field (26830 atoms), random
-non-overlapping placement in a 277 A periodic box (5 mM per chain), cut-off
-7.25 A, 300 K (T* 0.5962) with the Andersen thermostat, plain FP64 path
-(bit-exact with the reference).  A step is one call of the device event loop
-over `--pops` pops; bulk prediction (Predict(1..N) + CreateCBT) runs before
-the timed region.
+Workload (N=1): a seeded batch of 48 independent peptide systems of uneven
+size (chains of 10-80 residues, 1-32 chains per system, uniform-random
+sequences, about 455k atoms in all, 2.6k-23k per system), each in its own
+cubic periodic box at 5 mM per chain, Ding all-atom force field, cut-off
+7.25 A, 300 K (T* 0.5962) with the Andersen thermostat.  All 48 systems run
+in one persistent launch (one workgroup per system).
 
-Prints one JSON line.  `roofline` prices the dominant kernel (the persistent
-event-loop kernel) with algorithmic bytes: per pop N*88 B for the position
-and clock sweep plus (n_nb+1)*56 B + 40 B + 2*ceil(log2 N)*12 B per
-re-predicted atom.  `cpu_baseline` times the reference built from its own
-sources (oracle/_ref/sdmd_ref) on a bounded sample of the same input on one
-host core, or the CPU oracle when that binary is absent.
+A step is one launch of the device event loop on every system for a fixed
+window of device time (--step-ms); the launch ends with a drain that leaves
+every system at one consistent time.  `value` is the fast engine's executed
+events (stale pops excluded) summed over the systems, divided by the wall
+time of the timed steps.  The plain FP64 engine (bit-exact with the
+reference) runs the same systems for the same windows; `plain` and
+`vs_plain` report it.
+
+`roofline` prices the dominant kernel (the fast event loop) with algorithmic
+bytes per launch: every re-predicted atom reads its own state (96 B), the
+32-B FP32 mirror record of each neighbour-cell candidate, 56 B of FP64 state
+per FP64 re-solve, writes its 40-B event record and rescans one 64-atom block
+of event times (512 B); every executed event reads and writes the state of
+its atoms (2 x 104 B).  `cpu_baseline` times the reference built from its own
+sources (oracle/_ref/sdmd_ref) on one core, on a bounded sample (20000 pops)
+of the median-size system of the same batch.
 """
 import argparse
 import json
-import math
 import os
 import subprocess
 import sys
 import tempfile
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fast_event_loop_batch48_traffic.json")
 
 
-def build_workload(tmp, chains=64, length=40, seed=2, mM=5.0, cutoff=7.25, temperature=0.5962):
+def build_batch(tmp, nsys=48, seed=4, cutoff=7.25, temperature=0.5962):
     from sdmd_amd import builder
 
     t0 = time.time()
-    system = builder.random_peptides(chains, length, seed=seed, mM=mM)
-    builder.write_input_dir(system, tmp, cutoff=cutoff, temperature=temperature)
-    return system, time.time() - t0
+    workers = max(1, min(16, os.cpu_count() or 1))
+    def progress(done, total):
+        if done % 50 == 0 or done == total:
+            print("building chains: %d / %d" % (done, total), file=sys.stderr, flush=True)
+
+    systems = builder.batch_peptides(nsys=nsys, seed=seed, workers=workers, progress=progress)
+    dirs = []
+    for k, s in enumerate(systems):
+        d = os.path.join(tmp, "s%02d" % k)
+        builder.write_input_dir(s, d, cutoff=cutoff, temperature=temperature)
+        dirs.append(d)
+    return dirs, time.time() - t0
 
 
-def cpu_baseline(indir, pops, natom):
+def load_systems(dirs):
+    from sdmd_amd.engine import System
+
+    t0 = time.time()
+    with ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1))) as ex:
+        systems = list(ex.map(System.load, dirs))
+    return systems, time.time() - t0
+
+
+def cpu_baseline(indir, pops):
+    """The reference (own sources, -O2) on one core, or the CPU oracle."""
     ref = os.path.join(ROOT, "oracle", "_ref", "sdmd_ref")
     out = tempfile.mkdtemp(prefix="sdmd_cpu_")
     if os.path.exists(ref):
@@ -54,8 +83,8 @@ def cpu_baseline(indir, pops, natom):
         vals = dict(zip(tok[0::2], tok[1::2]))
         ev, sec = int(vals["events"]), float(vals["seconds"])
         return {"value": ev / sec, "unit": "events/s", "cores": 1, "kind": "reference",
-                "sample": "reference sDMD (own sources, -O2), same input, first %d pops (%d events) "
-                          "of its event loop, %.2f s, one core" % (pops, ev, sec)}
+                "sample": "reference sDMD (own sources, -O2) on %s, first %d pops (%d events) of its event "
+                          "loop, %.2f s, one core" % (os.path.basename(indir), pops, ev, sec)}
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     from sdmd_amd.engine import System
@@ -68,7 +97,47 @@ def cpu_baseline(indir, pops, natom):
     sec = time.perf_counter() - t0
     ev = int(o.counters()["per_type"].sum())
     return {"value": ev / sec, "unit": "events/s", "cores": 1, "kind": "port",
-            "sample": "CPU oracle, same input, first %d pops (%d events), %.2f s, one core" % (pops, ev, sec)}
+            "sample": "CPU oracle on %s, first %d pops (%d events), %.2f s, one core" % (
+                os.path.basename(indir), pops, ev, sec)}
+
+
+def totals(eng, n, fast):
+    t = {"frame": 0, "pops": 0, "recal": 0, "pair_evals": 0, "candidates": 0, "onebody": 0,
+         "screens": 0, "confirms": 0, "batches": 0, "causal": 0, "errors": set()}
+    for k in range(n):
+        st = eng.stats(k)
+        t["frame"] += st["frame"]
+        t["pops"] += st["pops"]
+        t["recal"] += st["recal"]
+        t["pair_evals"] += st["pair_evals"]
+        t["candidates"] += st["candidates"]
+        t["onebody"] += int(st["per_type"][4] + st["per_type"][10])     # thermostat, cell crossing
+        t["errors"].add(st["error"])
+        if fast:
+            fs = eng.fast_stats(k)
+            for f in ("screens", "confirms", "batches", "causal"):
+                t[f] += fs[f]
+    return t
+
+
+def diff(a, b):
+    return {k: (a[k] - b[k]) for k in a if k != "errors"}
+
+
+def timed(eng, nsys, fast, steps, warmup, step_ms):
+    for _ in range(warmup):
+        eng.run_for(1 << 40, step_ms)
+    t0s = totals(eng, nsys, fast)
+    t0 = time.perf_counter()
+    kernel_ms = 0.0
+    for _ in range(steps):
+        eng.run_for(1 << 40, step_ms)
+        kernel_ms += eng.last_kernel_ms()
+    elapsed = time.perf_counter() - t0
+    t1s = totals(eng, nsys, fast)
+    d = diff(t1s, t0s)
+    d["errors"] = t1s["errors"]
+    return d, elapsed, kernel_ms
 
 
 def calibrate_copy(nbytes=1 << 30):
@@ -86,70 +155,59 @@ def calibrate_copy(nbytes=1 << 30):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--pops", type=int, default=20000, help="event-loop pops per step")
-    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--step-ms", type=float, default=1000.0, help="device-time window of one step")
+    ap.add_argument("--block", type=int, default=512, help="fast engine threads per system")
+    ap.add_argument("--plain-block", type=int, default=256)
+    ap.add_argument("--systems", type=int, default=48)
     ap.add_argument("--cpu-pops", type=int, default=20000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--workdir", default=None, help="build the inputs here, or reuse them when present")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py measures one GPU: the boxes here have a single MI355X; "
-              "systems shard across GPUs by whole systems (see DESIGN.md), N=%d is not run" % args.gpus,
-              file=sys.stderr)
+        print("bench.py measures one GPU: the boxes here have a single MI355X; the batch shards "
+              "across GPUs by whole systems, N=%d is not run" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
         return
 
-    from sdmd_amd.engine import GpuEngine, System
+    from sdmd_amd.engine import GpuEngine
 
-    tmp = tempfile.mkdtemp(prefix="sdmd_bench_")
-    system, tbuild = build_workload(tmp)
-    t0 = time.time()
-    sysm = System.load(tmp)
-    thost = time.time() - t0
-    N = sysm.natom
-    total_pops = (args.steps + args.warmup) * args.pops
-    eng = GpuEngine([sysm], block=args.block, max_pops=total_pops + 100)
+    tmp = args.workdir or tempfile.mkdtemp(prefix="sdmd_bench_")
+    dirs = [os.path.join(tmp, "s%02d" % k) for k in range(args.systems)]
+    if all(os.path.exists(os.path.join(d, "coordinate.gro")) for d in dirs):
+        tbuild = 0.0                                   # inputs already built in --workdir
+    else:
+        os.makedirs(tmp, exist_ok=True)
+        dirs, tbuild = build_batch(tmp, nsys=args.systems)
+    systems, thost = load_systems(dirs)
+    nsys = len(systems)
+    natoms = [s.natom for s in systems]
+
+    # fast engine: FP32 screen + FP64 confirm, concurrent independent events
+    eng = GpuEngine(systems, block=args.block, mode="fast")
     eng.prepare()
     bulk_ms = eng.last_kernel_ms()
-    for _ in range(args.warmup):
-        eng.run(args.pops)
-
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:
-        sync = lambda: None  # noqa: E731
-    s0 = eng.stats(0)
-    sync()
-    t0 = time.perf_counter()
-    kernel_ms = 0.0
-    for _ in range(args.steps):
-        eng.run(args.pops)
-        kernel_ms += eng.last_kernel_ms()
-    sync()
-    elapsed = time.perf_counter() - t0
-    s1 = eng.stats(0)
-    assert s1["error"] == 0
-    events = s1["frame"] - s0["frame"]
-    pops = s1["pops"] - s0["pops"]
-    recal = s1["recal"] - s0["recal"]
-    evals = s1["pair_evals"] - s0["pair_evals"]
-    cands = s1["candidates"] - s0["candidates"]
-    renewed = 2 * events + recal - (s1["per_type"] - s0["per_type"])[[4, 10]].sum()   # one-body events renew 1 atom
-    n_nb = cands / max(renewed, 1)
-    logn = math.ceil(math.log2(N))
-    bytes_total = pops * N * 88 + renewed * ((n_nb + 1) * 56 + 40 + 2 * logn * 12)
-    achieved = bytes_total / (kernel_ms / 1e3) / 1e9
+    f, elapsed, kernel_ms = timed(eng, nsys, True, args.steps, args.warmup, args.step_ms)
+    assert f["errors"] == {0}, f["errors"]
+    eng.close()
+    events = f["frame"]
+    predictions = 2 * events + f["recal"] - f["onebody"]
+    n_nb = f["candidates"] / max(predictions, 1)
+    bytes_launch = (predictions * (96 + n_nb * 32 + 40 + 512) + f["confirms"] * 56 + events * 2 * 104) / args.steps
+    kernel_ms_launch = kernel_ms / args.steps
+    achieved = bytes_launch / (kernel_ms_launch / 1e3) / 1e9
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", "plain_event_loop_cfg2_traffic.json")
-    if os.path.exists(pmc):
-        with open(pmc) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            per_event = json.load(fh).get("hbm_bytes_per_event")
+        if per_event:
+            traffic = per_event * events / args.steps
 
     line = {
         "metric": "aggregate events/s + pair-time evals/s; batched FP32-screen vs plain FP64 GPU",
@@ -162,25 +220,39 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32/f64",
         "data": "synthetic (seeded builder: random sequences, random placement)",
-        "config": {"workload": "64 x 40-residue random peptides, %d atoms, 5 mM, %.1f A box, cutoff 7.25 A, "
-                               "300 K Andersen, plain FP64 path" % (N, system.box[0]),
-                   "natom": N, "pops_per_step": args.pops, "block": args.block},
-        "pair_evals_per_s": evals / elapsed,
-        "pops_per_s": pops / elapsed,
-        "events": events,
+        "config": {"workload": "batch of %d peptide systems, 10-80 residues x 1-32 chains each, %d atoms "
+                               "(%d-%d per system), 5 mM per chain, cut-off 7.25 A, 300 K Andersen; "
+                               "fast engine (FP32 screen + FP64 confirm)" % (nsys, sum(natoms), min(natoms),
+                                                                             max(natoms)),
+                   "systems": nsys, "natom_total": sum(natoms), "block": args.block, "step_ms": args.step_ms},
+        "pair_evals_per_s": (f["screens"] + f["pair_evals"]) / elapsed,
+        "fp32_screens_per_s": f["screens"] / elapsed,
+        "fp64_pair_evals_per_s": f["pair_evals"] / elapsed,
+        "pops_per_s": f["pops"] / elapsed,
+        "events_per_round": f["pops"] / max(f["batches"], 1),
         "n_nb": n_nb,
+        "causal_reads": f["causal"],
         "bulk_predict_ms": bulk_ms,
         "setup_s": {"build": tbuild, "host": thost},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "bytes_per_launch": bytes_total / args.steps,
-                     "kernel_ms_per_launch": kernel_ms / args.steps},
+                     "bytes_per_launch": bytes_launch, "kernel_ms_per_launch": kernel_ms_launch},
     }
+    if not args.no_plain:
+        pops_cap = int(40000 * (args.steps + args.warmup) * args.step_ms / 1000.0) + 1000
+        eng = GpuEngine(systems, block=args.plain_block, mode="plain", max_pops=pops_cap)
+        eng.prepare()
+        p, pel, pkms = timed(eng, nsys, False, args.steps, args.warmup, args.step_ms)
+        assert p["errors"] == {0}, p["errors"]
+        eng.close()
+        line["plain"] = {"value": p["frame"] / pel, "unit": "events/s", "pair_evals_per_s": p["pair_evals"] / pel,
+                         "pops_per_s": p["pops"] / pel, "block": args.plain_block}
+        line["vs_plain"] = line["value"] / line["plain"]["value"]
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(tmp, args.cpu_pops, N)
-    eng.close()
+        order = sorted(range(nsys), key=lambda k: natoms[k])
+        line["cpu_baseline"] = cpu_baseline(dirs[order[nsys // 2]], args.cpu_pops)
     print(json.dumps(line))
 
 

@@ -137,6 +137,16 @@ int sdmd_gpu_run_for(sdmd_gpu_ctx *ctx, int64_t max_pops_per_system, double max_
  * max_pops pops per system. */
 int sdmd_gpu_run_until(sdmd_gpu_ctx *ctx, double t_end, int64_t max_pops_per_system);
 int sdmd_gpu_fast_stats_get(sdmd_gpu_ctx *ctx, int32_t sys, sdmd_gpu_fast_stats *out);
+/* Fast mode, batching form: time every workgroup shape (4, 8 or 16
+ * wavefronts per system, i.e. concurrent events per round and pass, and 2 or
+ * 4 candidates staged per lane) for trial_ms of device time on the live
+ * systems and keep the fastest; the systems advance meanwhile.  No reference
+ * counterpart (the reference is single-threaded). */
+int sdmd_gpu_autotune(sdmd_gpu_ctx *ctx, double trial_ms, int32_t *block_out, int32_t *depth_out);
+/* Fast mode: `launches` event-loop launches of max_ms device time each,
+ * captured once into a HIP graph and replayed (re-captured when the shape or
+ * arguments change).  No reference counterpart. */
+int sdmd_gpu_run_graph(sdmd_gpu_ctx *ctx, int32_t launches, double max_ms);
 
 int sdmd_gpu_predict(sdmd_gpu_ctx *ctx, int32_t sys, const int32_t *renew);
 int sdmd_gpu_create_tree(sdmd_gpu_ctx *ctx, int32_t sys);

@@ -60,7 +60,8 @@ GPU_SYMBOLS = ["sdmd_gpu_open", "sdmd_gpu_close", "sdmd_gpu_set_normals", "sdmd_
                "sdmd_gpu_pop", "sdmd_gpu_advance", "sdmd_gpu_do_event", "sdmd_gpu_cell_move",
                "sdmd_gpu_download", "sdmd_gpu_get_tree", "sdmd_gpu_stats_get", "sdmd_gpu_get_trace",
                "sdmd_gpu_get_hashes", "sdmd_gpu_phase_times", "sdmd_gpu_last_kernel_ms",
-               "sdmd_gpu_run_for", "sdmd_gpu_run_until", "sdmd_gpu_fast_stats_get"]
+               "sdmd_gpu_run_for", "sdmd_gpu_run_until", "sdmd_gpu_fast_stats_get", "sdmd_gpu_autotune",
+               "sdmd_gpu_run_graph"]
 
 
 def lib():
@@ -92,6 +93,8 @@ def lib():
     L.sdmd_gpu_run.argtypes = [P, I64]
     L.sdmd_gpu_run_for.argtypes = [P, I64, C.c_double]
     L.sdmd_gpu_run_until.argtypes = [P, C.c_double, I64]
+    L.sdmd_gpu_autotune.argtypes = [P, C.c_double, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.sdmd_gpu_run_graph.argtypes = [P, C.c_int32, C.c_double]
     L.sdmd_gpu_fast_stats_get.argtypes = [P, C.c_int32, C.POINTER(FastStats)]
     L.sdmd_gpu_predict.argtypes = [P, I, C.c_void_p]
     L.sdmd_gpu_create_tree.argtypes = [P, I]

@@ -546,19 +546,30 @@ def _build_one(args):
         return build_chain(seq, np.random.default_rng(seed), extended=extended)
 
 
-def build_chains(seqs, seed, extended=False, workers=None):
+def build_chains(seqs, seed, extended=False, workers=None, progress=None):
     """Build chains in parallel; chain k uses its own seed (seed, k), so the
-    result does not depend on worker scheduling."""
+    result does not depend on worker scheduling.  `progress(done, total)` is
+    called as chains complete."""
     for aa in set(a for sq in seqs for a in sq) | {"ALA"}:
         _conformer(aa)
     jobs = [(list(sq), [seed, k], extended) for k, sq in enumerate(seqs)]
     if workers is None:
         workers = min(len(jobs), max(1, min(16, os.cpu_count() or 1)))
     if workers <= 1 or len(jobs) == 1:
-        return [_build_one(j) for j in jobs]
+        out = []
+        for j in jobs:
+            out.append(_build_one(j))
+            if progress:
+                progress(len(out), len(jobs))
+        return out
     import multiprocessing as mp
     with mp.get_context("fork").Pool(workers) as pool:
-        return pool.map(_build_one, jobs, chunksize=1)
+        out = []
+        for ch in pool.imap(_build_one, jobs, chunksize=1):
+            out.append(ch)
+            if progress:
+                progress(len(out), len(jobs))
+        return out
 
 
 def random_sequences(nchains, length, seed):
@@ -598,7 +609,7 @@ def batch_specs(nsys=48, seed=4, length=(10, 80), chains=(1, 32), min_atoms=2000
     return out
 
 
-def batch_peptides(nsys=48, seed=4, mM=5.0, workers=None, **kw):
+def batch_peptides(nsys=48, seed=4, mM=5.0, workers=None, progress=None, **kw):
     """A batch of independent peptide systems of uneven size (each its own
     cubic periodic box at mM per chain, widened where a chain's extent needs
     it: edge >= 2*extent + 8 A).  Every chain of every system is built in
@@ -609,7 +620,7 @@ def batch_peptides(nsys=48, seed=4, mM=5.0, workers=None, **kw):
         s = random_sequences(nc, L, seed * 1000 + k)
         seqs += s
         owner += [k] * len(s)
-    chains = build_chains(seqs, seed, workers=workers)
+    chains = build_chains(seqs, seed, workers=workers, progress=progress)
     systems = []
     for k, (nc, L) in enumerate(specs):
         mine = [c for c, o in zip(chains, owner) if o == k]

@@ -99,6 +99,8 @@ struct sdmd_host_system {
     /* parameters */
     char neworcontinue[20], method[20], thermostat[20], temptype[20];
     char wallexist[20], walltype[20];
+    int wall_type;                 /* ReadWall: AtomModel of Wall.txt's type name */
+    double wall_mass, wall_pos[4];
     double timestep, outputrate, cutoffr, thermoF, T;
     /* atoms */
     int natom, naa, nprot;
@@ -890,6 +892,28 @@ static void generate_velocity(sdmd_host_system *s) {
     free(com);
 }
 
+/* ReadWall, Initialization.c:1893-1935: the smooth wall is one pseudo-atom
+ * whose type and mass come from Library_Ding/Wall.txt ("Wall", then "ATOM
+ * name type mass extra extra"), placed at the box centre with x = 0 for a
+ * cylinder (the axis runs along x) */
+static int read_wall(sdmd_host_system *s) {
+    char path[1200], line[512], tag[64], name[64], rname[64], e0[64], e1[64];
+    snprintf(path, sizeof path, "%s/Library_%s/Wall.txt", s->datadir, s->method);
+    FILE *f = fopen(path, "r");
+    if (!f) return fail(s, SDMD_EIO, "cannot open %s", path);
+    double mass = 0;
+    int ok = fgets(line, sizeof line, f) != NULL &&
+             fscanf(f, "%63s%63s%63s%lf%63s%63s", tag, name, rname, &mass, e0, e1) == 6;
+    fclose(f);
+    if (!ok) return fail(s, SDMD_EFORMAT, "%s: expected a header line and an ATOM line", path);
+    s->wall_type = atom_model(rname);
+    if (s->wall_type <= 0) return fail(s, SDMD_EFORMAT, "%s: unknown wall atom type %s", path, rname);
+    s->wall_mass = mass;
+    for (int k = 1; k <= 3; k++) s->wall_pos[k] = s->box[k] / 2;
+    s->wall_pos[1] = 0;                /* cylinder: the wall atom sits on the x axis */
+    return SDMD_OK;
+}
+
 int sdmd_host_load(const char *datadir, const char *gro_name, uint32_t seed,
                    sdmd_host_system **out, char *err, int errlen) {
     if (!datadir || !gro_name || !out) return SDMD_EINVAL;
@@ -909,6 +933,7 @@ int sdmd_host_load(const char *datadir, const char *gro_name, uint32_t seed,
     if ((rc = read_model(s))) goto bad;
     if ((rc = create_potential_matrix(s))) goto bad;
     if ((rc = read_hb(s))) goto bad;
+    if (strcmp(s->wallexist, "no") && (rc = read_wall(s))) goto bad;
     for (int i = 1; i <= s->natom; i++) hb_neighbor_assign(s, i);
     generate_velocity(s);
     *out = s;
@@ -1286,6 +1311,9 @@ const sdmd_system_desc *sdmd_host_desc(sdmd_host_system *s) {
     d->thermo_freq = s->thermoF;
     d->thermostat = strcmp(s->thermostat, "no") ? 1 : 0;
     d->wall = strcmp(s->wallexist, "no") ? 1 : 0;
+    d->wall_type = s->wall_type;
+    d->wall_mass = s->wall_mass;
+    for (int k = 0; k < 3; k++) d->wall_pos[k] = s->wall_pos[k + 1];
     d->nprotein = s->nprot;
 #define ARR(T, n) ((T *)flat_alloc(s, sizeof(T) * (size_t)(n)))
     d->x = ARR(double, N1); d->y = ARR(double, N1); d->z = ARR(double, N1);

@@ -382,7 +382,7 @@ struct PosStored {
 };
 
 template <class P>
-static __device__ void image_near(const Sys &S, const P &V, int i, int j, double *p) {
+static __device__ __forceinline__ void image_near(const Sys &S, const P &V, int i, int j, double *p) {
     const int ci[3] = {G(S.cx)[i], G(S.cy)[i], G(S.cz)[i]}, cj[3] = {G(S.cx)[j], G(S.cy)[j], G(S.cz)[j]};
     V.pos(j, p);
     for (int n = 0; n < 3; n++) {
@@ -395,7 +395,7 @@ static __device__ void image_near(const Sys &S, const P &V, int i, int j, double
 
 // HBEnergyChange, Events.c:1027-1095 (the accumulated value persists across candidates)
 template <class P>
-static __device__ double hb_energy_change(const Sys &S, const P &V, int t, int skip, int hbtype, int *err) {
+static __device__ __forceinline__ double hb_energy_change(const Sys &S, const P &V, int t, int skip, int hbtype, int *err) {
     const double cut = S.cutoffr * S.cutoffr;
     double energy = 0, acc = 0;
     for (int c = 0; c < 27; c++) {
@@ -431,7 +431,7 @@ static __device__ double hb_energy_change(const Sys &S, const P &V, int t, int s
 
 // InteractionEvent, Events.c:798-1024
 template <class P>
-static __device__ void interaction_event(Sys &S, const P &V, int ti, int pj, int nbi, int nbj, int *err) {
+static __device__ __forceinline__ void interaction_event(Sys &S, const P &V, int ti, int pj, int nbi, int nbj, int *err) {
     double pti[3];
     V.pos(ti, pti);
     const double pi0 = pti[0], pi1 = pti[1], pi2 = pti[2];
@@ -591,4 +591,6 @@ int sdmd_fast_run(sdmd_gpu_ctx *c, long long pops, double max_ms, double t_end);
 int sdmd_fast_download(sdmd_gpu_ctx *c, int s, sdmd_system_desc *out);
 int sdmd_fast_stats(sdmd_gpu_ctx *c, int s, sdmd_gpu_fast_stats *out);
 int sdmd_fast_sync(sdmd_gpu_ctx *c);
+int sdmd_fast_run_graph(sdmd_gpu_ctx *c, int launches, double max_ms);
+int sdmd_fast_autotune(sdmd_gpu_ctx *c, double trial_ms, int32_t *block_out, int32_t *depth_out);
 void sdmd_fast_close(sdmd_gpu_ctx *c);

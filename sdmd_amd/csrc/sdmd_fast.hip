@@ -697,12 +697,13 @@ __device__ __forceinline__ void screen_at(const FSys &F, const ScrTab &T, const 
     o.t = sc.t; o.e = sc.e; o.flag = sc.flag || (rr > calib - cband);
 }
 
-constexpr int FQ = 4;     // candidates per lane held in registers (256 per wave)
+// FQ: candidates per lane held in registers (64*FQ per wave in one pass)
 
 // screened CollisionTime: FP32 screen of the neighbourhood, FP64 re-solve of
 // the marked candidates and of those whose interval reaches the smallest
 // upper bound; the FP64 minimum (first in scan order on ties) competes with
 // the running best under JobAssign's strict '<'
+template <int FQ>
 __device__ __forceinline__ void collision_fast(FSys &F, const ScrTab &T, FWave &wm, const FAtom &A, Best &best, long long &nscreen,
                                long long &nconf, int *err) {
     const Sys &S = F.s;
@@ -903,7 +904,8 @@ __device__ double hbfree64(const FSys &F, const FAtom &A, int j, double calib, d
 }
 
 // Predict for atom i at time t (its state is at t); writes the event record
-__device__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm, int i, double t, WStats &ws, int *err) {
+template <int FQ>
+__device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm, int i, double t, WStats &ws, int *err) {
     Sys &S = F.s;
     const int lane = threadIdx.x & 63;
     long long &nscreen = ws.screen, &nconf = ws.conf, &nfp64 = ws.fp64;
@@ -917,12 +919,8 @@ __device__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm, int i, double 
     if (nb > 16 && lane == 0) *err = ERR_CELLCAP;      // BondTime's connect[16], TimePrediction.c:188
     Best init;
     init.t = INFINIT; init.pot = 0; init.pc = 0; init.partner = INVALID; init.type = SDMD_INVD; init.sub = 0;
-    // CollisionTime (screened), against no running best yet
-    Best cw = init;
-    cw.t = __builtin_inf();
-    collision_fast(F, T, wm, A, cw, nscreen, nconf, err);
-    if (S.prof) pt = wall_clock64();
-    // BondTime: one lane per bond (lanes 32..), first minimal wins
+    // BondTime: one lane per bond (lanes 32..), first minimal wins; done
+    // before the screen so the staged partner state is dead during it
     Best best = init;
     {
         Best mine = init;
@@ -939,9 +937,14 @@ __device__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm, int i, double 
         best = out;
         nfp64 += (lane == 0) ? nb : 0;
     }
+    PROF_MARK(S, FPH_BOND, pt);
+    // CollisionTime (screened), against no running best yet
+    Best cw = init;
+    cw.t = __builtin_inf();
+    collision_fast<FQ>(F, T, wm, A, cw, nscreen, nconf, err);
+    if (S.prof) pt = wall_clock64();
     // JobAssign order: bonds first, then the collision winner if strictly earlier
     if (cw.partner > 0 && cw.t > 0 && best.t > cw.t) best = cw;
-    PROF_MARK(S, FPH_BOND, pt);
     // PBCandCrossCellTime, TimePrediction.c:598-632
     {
         double t_ev = INFINIT;
@@ -1141,7 +1144,7 @@ __device__ void thermostat_fast(FSys &F, int i) {
 
 // execute one popped event on lane 0 (DoEvent, Events.c:25-114, with the
 // counter increments of SGThread.c:100-104); returns the renew count
-__device__ int exec_fast(FSys &F, int a, double t, int *renew, int *err) {
+__device__ __forceinline__ int exec_fast(FSys &F, int a, double t, int *renew, int *err) {
     Sys &S = F.s;
     const int p = G(S.evp)[a];
     const int partner = (p > 0 && p <= S.N) ? p : 0;
@@ -1291,7 +1294,7 @@ __global__ __launch_bounds__(64 * W) void k_fast_prepare(FSys *systems) {
     WStats ws = {0, 0, 0, 0};
     int err = 0;
     for (int i = wave + 1; i <= S.N; i += W) {
-        predict_fast(F, tab, sh.w[wave], i, G(F.t0)[i], ws, &err);
+        predict_fast<4>(F, tab, sh.w[wave], i, G(F.t0)[i], ws, &err);
         if (__ballot(err != 0)) break;              // uniform across the wave
     }
     if (err) atomicCAS(&sh.err, 0, err);
@@ -1315,12 +1318,13 @@ __global__ __launch_bounds__(64 * W) void k_fast_prepare(FSys *systems) {
 
 // the event loop: rounds of concurrent independent events until `npops`
 // pops are done or the device clock passes `budget` ticks (100 MHz)
-template <int W>
-__global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long npops, long long budget) {
+template <int W, int FQ>
+__global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long npops, long long budget, double t_end) {
     __shared__ FSys Fl;
     __shared__ FShared sh;
     __shared__ ScrTab tab;
     fsys_load<W>(Fl, &systems[blockIdx.x]);
+    if (threadIdx.x == 0) Fl.t_stop = t_end;
     tab_load<W>(tab, Fl.scrtab);
     FSys &F = Fl;
     Sys &S = F.s;
@@ -1527,7 +1531,7 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
         err = 0;
         for (int r = wave; r < nren && sh.err == 0; r += W) {
             const int a = sh.ren_a[r];
-            predict_fast(F, tab, sh.w[wave], a, sh.ren_t[r], ws, &err);
+            predict_fast<FQ>(F, tab, sh.w[wave], a, sh.ren_t[r], ws, &err);
             if (__ballot(err != 0)) break;          // uniform across the wave
         }
         for (int r = tid; r < nren; r += 64 * W) sh.dirty[r] = (sh.ren_a[r] - 1) >> 6;
@@ -1577,8 +1581,13 @@ using namespace sdmd;
 struct sdmd::FastState {
     FSys *host;
     FSys *dev;
-    int waves;
+    int waves;                 // wavefronts per workgroup (events in flight per round per pass)
+    int depth;                 // candidates per lane staged in registers (FQ)
     int validate;
+    hipGraph_t graph;          // captured launch sequence (sdmd_gpu_run_graph)
+    hipGraphExec_t gexec;
+    int g_launches;
+    long long g_budget;
 };
 
 template <typename T>
@@ -1591,6 +1600,7 @@ int sdmd_fast_open(sdmd_gpu_ctx *c, const sdmd_gpu_opts *o) {
     if (!fs) return SDMD_ENOMEM;
     c->fast = fs;
     fs->waves = c->block / 64;
+    fs->depth = 4;
     if (fs->waves < 1 || fs->waves > FMAXW) return SDMD_EINVAL;
     fs->validate = o ? o->validate : 0;
     fs->host = (FSys *)calloc((size_t)c->nsys, sizeof(FSys));
@@ -1708,6 +1718,41 @@ int sdmd_fast_sync(sdmd_gpu_ctx *c) { return fast_sync_from(c); }
     case 16: hipLaunchKernelGGL(kern<16>, dim3(c->nsys), dim3(1024), 0, c->stream, __VA_ARGS__); break; \
     default: hipLaunchKernelGGL(kern<1>, dim3(c->nsys), dim3(64), 0, c->stream, __VA_ARGS__); break; } } while (0)
 
+// the event loop for the context's shape (wavefronts x staging depth)
+static void launch_run(sdmd_gpu_ctx *c, long long pops, long long budget, double t_end) {
+    FastState *fs = c->fast;
+    const dim3 g(c->nsys);
+#define FRUN(Wv, Qv) hipLaunchKernelGGL((k_fast_run<Wv, Qv>), g, dim3(64 * Wv), 0, c->stream, fs->dev, pops, budget, t_end)
+    if (fs->depth == 2) {
+        switch (fs->waves) {
+        case 4: FRUN(4, 2); break;
+        case 8: FRUN(8, 2); break;
+        case 16: FRUN(16, 2); break;
+        default: FRUN(1, 2); break;
+        }
+    } else {
+        switch (fs->waves) {
+        case 4: FRUN(4, 4); break;
+        case 8: FRUN(8, 4); break;
+        case 16: FRUN(16, 4); break;
+        default: FRUN(1, 4); break;
+        }
+    }
+#undef FRUN
+}
+
+static int budget_ticks(double max_ms, long long *out) {
+    *out = 1LL << 62;
+    if (max_ms > 0) {
+        int dev = 0, khz = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+            return SDMD_EDEVICE;
+        *out = (long long)(max_ms * khz);
+    }
+    return SDMD_OK;
+}
+
 int sdmd_fast_prepare(sdmd_gpu_ctx *c) {
     FastState *fs = c->fast;
     HIPV(hipEventRecord(c->ev0, c->stream));
@@ -1725,19 +1770,10 @@ int sdmd_fast_prepare(sdmd_gpu_ctx *c) {
 
 int sdmd_fast_run(sdmd_gpu_ctx *c, long long pops, double max_ms, double t_end) {
     FastState *fs = c->fast;
-    for (int s = 0; s < c->nsys; s++) fs->host[s].t_stop = t_end;
-    if (hipMemcpyAsync(fs->dev, fs->host, sizeof(FSys) * c->nsys, hipMemcpyHostToDevice, c->stream) != hipSuccess)
-        return SDMD_EDEVICE;
-    long long budget = 1LL << 62;
-    if (max_ms > 0) {
-        int dev = 0, khz = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
-            return SDMD_EDEVICE;
-        budget = (long long)(max_ms * khz);
-    }
+    long long budget;
+    if (budget_ticks(max_ms, &budget)) return SDMD_EDEVICE;
     HIPV(hipEventRecord(c->ev0, c->stream));
-    FDISPATCH(k_fast_run, fs->dev, pops, budget);
+    launch_run(c, pops, budget, t_end);
     HIPV(hipEventRecord(c->ev1, c->stream));
     if (hipStreamSynchronize(c->stream) != hipSuccess) return SDMD_EDEVICE;
     float ms = 0;
@@ -1825,8 +1861,68 @@ int sdmd_fast_stats(sdmd_gpu_ctx *c, int s, sdmd_gpu_fast_stats *out) {
     return SDMD_OK;
 }
 
+// A sequence of `launches` event-loop launches (each bounded by max_ms of
+// device time) captured once into a HIP graph and replayed; the graph is
+// rebuilt when the shape, count or bound changes.
+int sdmd_fast_run_graph(sdmd_gpu_ctx *c, int launches, double max_ms) {
+    FastState *fs = c->fast;
+    long long budget;
+    if (launches < 1 || budget_ticks(max_ms, &budget)) return SDMD_EINVAL;
+    const int key = launches * 64 + fs->waves * 4 + fs->depth;
+    if (!fs->gexec || fs->g_launches != key || fs->g_budget != budget) {
+        if (fs->gexec) { HIPV(hipGraphExecDestroy(fs->gexec)); fs->gexec = nullptr; }
+        if (fs->graph) { HIPV(hipGraphDestroy(fs->graph)); fs->graph = nullptr; }
+        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return SDMD_EDEVICE;
+        for (int k = 0; k < launches; k++) launch_run(c, 1LL << 60, budget, INFINIT);
+        if (hipStreamEndCapture(c->stream, &fs->graph) != hipSuccess) return SDMD_EDEVICE;
+        if (hipGraphInstantiate(&fs->gexec, fs->graph, nullptr, nullptr, 0) != hipSuccess) return SDMD_EDEVICE;
+        fs->g_launches = key;
+        fs->g_budget = budget;
+    }
+    HIPV(hipEventRecord(c->ev0, c->stream));
+    if (hipGraphLaunch(fs->gexec, c->stream) != hipSuccess) return SDMD_EDEVICE;
+    HIPV(hipEventRecord(c->ev1, c->stream));
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return SDMD_EDEVICE;
+    float ms = 0;
+    HIPV(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    if (fast_sync_from(c)) return SDMD_EDEVICE;
+    for (int s = 0; s < c->nsys; s++)
+        if (fs->host[s].s.error) return SDMD_ESTATE;
+    return SDMD_OK;
+}
+
+// Time each workgroup shape (wavefronts per system x candidates staged per
+// lane) on the live systems for trial_ms of device time and keep the one that
+// executes the most events per millisecond; the systems advance meanwhile.
+int sdmd_fast_autotune(sdmd_gpu_ctx *c, double trial_ms, int32_t *block_out, int32_t *depth_out) {
+    FastState *fs = c->fast;
+    static const int shapes[][2] = {{4, 4}, {8, 4}, {16, 4}, {4, 2}, {8, 2}, {16, 2}};
+    double best = -1;
+    int bw = fs->waves, bq = fs->depth;
+    for (const auto &sh : shapes) {
+        fs->waves = sh[0];
+        fs->depth = sh[1];
+        long long ev0 = 0, ev1 = 0;
+        for (int s = 0; s < c->nsys; s++) ev0 += fs->host[s].s.frame;
+        int rc = sdmd_fast_run(c, 1LL << 60, trial_ms, INFINIT);
+        if (rc) return rc;
+        for (int s = 0; s < c->nsys; s++) ev1 += fs->host[s].s.frame;
+        const double rate = (double)(ev1 - ev0) / (c->last_ms > 0 ? c->last_ms : 1.0);
+        if (rate > best) { best = rate; bw = sh[0]; bq = sh[1]; }
+    }
+    fs->waves = bw;
+    fs->depth = bq;
+    c->block = 64 * bw;
+    if (block_out) *block_out = 64 * bw;
+    if (depth_out) *depth_out = bq;
+    return SDMD_OK;
+}
+
 void sdmd_fast_close(sdmd_gpu_ctx *c) {
     if (!c->fast) return;
+    if (c->fast->gexec) HIPV(hipGraphExecDestroy(c->fast->gexec));
+    if (c->fast->graph) HIPV(hipGraphDestroy(c->fast->graph));
     free(c->fast->host);
     free(c->fast);
     c->fast = nullptr;

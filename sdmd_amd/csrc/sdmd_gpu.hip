@@ -1278,3 +1278,15 @@ extern "C" int sdmd_gpu_fast_stats_get(sdmd_gpu_ctx *c, int32_t s, sdmd_gpu_fast
     memset(out, 0, sizeof *out);
     return sdmd_fast_stats(c, s, out);
 }
+
+extern "C" int sdmd_gpu_autotune(sdmd_gpu_ctx *c, double trial_ms, int32_t *block_out, int32_t *depth_out) {
+    if (!c || !(trial_ms > 0)) return SDMD_EINVAL;
+    if (c->mode != SDMD_MODE_FAST) return SDMD_EUNSUPPORTED;
+    return sdmd_fast_autotune(c, trial_ms, block_out, depth_out);
+}
+
+extern "C" int sdmd_gpu_run_graph(sdmd_gpu_ctx *c, int32_t launches, double max_ms) {
+    if (!c || launches < 1 || !(max_ms > 0)) return SDMD_EINVAL;
+    if (c->mode != SDMD_MODE_FAST) return SDMD_EUNSUPPORTED;
+    return sdmd_fast_run_graph(c, launches, max_ms);
+}

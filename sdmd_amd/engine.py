@@ -151,6 +151,18 @@ class GpuEngine:
         """Run until the clock passes t_end (at most max_pops pops per system)."""
         check(_lib.lib().sdmd_gpu_run_until(self._h, float(t_end), int(max_pops)), "sdmd_gpu_run_until")
 
+    def autotune(self, trial_ms=200.0):
+        """Fast mode: pick the fastest workgroup shape on the live systems;
+        returns (block, staging depth)."""
+        b, q = C.c_int32(), C.c_int32()
+        check(_lib.lib().sdmd_gpu_autotune(self._h, float(trial_ms), C.byref(b), C.byref(q)), "sdmd_gpu_autotune")
+        self.block = b.value
+        return b.value, q.value
+
+    def run_graph(self, launches, max_ms):
+        """Fast mode: `launches` bounded launches replayed from a captured HIP graph."""
+        check(_lib.lib().sdmd_gpu_run_graph(self._h, int(launches), float(max_ms)), "sdmd_gpu_run_graph")
+
     def fast_stats(self, sys=0):
         st = FastStats()
         check(_lib.lib().sdmd_gpu_fast_stats_get(self._h, sys, C.byref(st)), "sdmd_gpu_fast_stats_get")
