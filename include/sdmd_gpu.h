@@ -136,6 +136,22 @@ int sdmd_gpu_run_for(sdmd_gpu_ctx *ctx, int64_t max_pops_per_system, double max_
  * t_end; fast mode executes every event with time <= t_end), at most
  * max_pops pops per system. */
 int sdmd_gpu_run_until(sdmd_gpu_ctx *ctx, double t_end, int64_t max_pops_per_system);
+/* Run until every system has executed `events` more events (stale pops do
+ * not count; fast mode finishes with its closing drain, so a system may
+ * execute a few more), or max_ms of device time has passed (<= 0: no limit).
+ * The REMD loop's exchange interval (REMDRun, REMD.c:38-118, where the
+ * reference counts time units instead). */
+int sdmd_gpu_run_events(sdmd_gpu_ctx *ctx, int64_t events_per_system, double max_ms);
+/* CalKinetE and CalPotenE (ToolFunctions.c:626-752) of every system on the
+ * device, written to kinetic[nsys] and potential[nsys]: plain mode at the
+ * current time, fast mode at the snapshot time sdmd_gpu_download reports.
+ * The sums run in another order than the reference's loop (last-bit
+ * differences); the energy that REMD exchanges compare (REMD.c:97). */
+int sdmd_gpu_energy(sdmd_gpu_ctx *ctx, double *kinetic, double *potential);
+/* Set one system's thermostat temperature (reduced units, k_B T in
+ * kcal/mol) -- `targetTemperature = REMD_Temperature.T` after an exchange,
+ * REMD.c:100; the Andersen draws use variance T/m (ToolFunctions.c:605). */
+int sdmd_gpu_set_temperature(sdmd_gpu_ctx *ctx, int32_t sys, double temperature);
 int sdmd_gpu_fast_stats_get(sdmd_gpu_ctx *ctx, int32_t sys, sdmd_gpu_fast_stats *out);
 /* Fast mode, batching form: time every workgroup shape (4, 8 or 16
  * wavefronts per system, i.e. concurrent events per round and pass, and 2 or

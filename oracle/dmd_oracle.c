@@ -42,6 +42,9 @@ struct oracle {
     sdmd_pair hbpair[SDMD_NHBCLASS * NT * NT];
     double *sd, *se, *sacc;
     double hbpot[6];
+    /* smooth cylinder wall pseudo-atom, ReadWall Initialization.c:1907-1935 */
+    int wall, wall_type;
+    double wall_mass, wallpos[4];
     /* CBT, CBT.c */
     int *node, *leaf, count;
     /* clock and counters */
@@ -109,6 +112,10 @@ oracle *oracle_open(const sdmd_system_desc *d) {
     DUP(o->se, d->step_e, double, d->nstep_pool + 1);
     DUP(o->sacc, d->step_acc, double, d->nstep_pool + 1);
     memcpy(o->hbpot, d->hb_pot, sizeof o->hbpot);
+    o->wall = d->wall;
+    o->wall_type = d->wall_type;
+    o->wall_mass = d->wall_mass;
+    for (int k = 0; k < 3; k++) o->wallpos[k + 1] = d->wall_pos[k];
     o->node = calloc((size_t)2 * N + 2, sizeof(int));
     o->leaf = calloc((size_t)N + 2, sizeof(int));
     o->currenttime = d->currenttime;
@@ -251,12 +258,12 @@ static double hb_barrier(const oracle *o, int a1, int a2) {   /* HBBarrier, Even
 /* ---------------- FindPair, ToolFunctions.c:39-224 ---------------- */
 typedef struct { double lo, hi, lopot, hipot, acc; } fp_t;
 
+static int shell_lookup(oracle *o, const sdmd_pair *c, int kind, double direction, double d2, fp_t *f,
+                        int a1, int a2);
+
 static int find_pair(oracle *o, int a1, int a2, int kind, double direction, double d2, fp_t *f,
                      int hbpartner, int typechange) {
     const sdmd_pair *c = NULL;
-    int cs = -1;
-    direction = (direction < 0) ? 1 : -1;
-    double tol = direction * ZERO;
     if (kind == K_COLLISION) {
         int t1 = o->type[a1];
         if (typechange == 1) t1 = type_change(t1, 1);
@@ -273,6 +280,15 @@ static int find_pair(oracle *o, int a1, int a2, int kind, double direction, doub
         int t1 = type_change(o->type[a1], 0), t2 = type_change(o->type[a2], 0);
         c = &o->hbpair[(t * NT + t1) * NT + t2];
     }
+    return shell_lookup(o, c, kind, direction, d2, f, a1, a2);
+}
+
+/* the shell search of FindPair (ToolFunctions.c:104-224) on a resolved pair */
+static int shell_lookup(oracle *o, const sdmd_pair *c, int kind, double direction, double d2, fp_t *f,
+                        int a1, int a2) {
+    int cs = -1;
+    direction = (direction < 0) ? 1 : -1;
+    double tol = direction * ZERO;
     if (!c || c->dmin == 0) { o->error = 1; f->lo = f->hi = f->acc = INFINIT; f->lopot = f->hipot = 0; return -1; }
     int flag = (c->nstep == 0);
     if (d2 < c->dmin - ZERO) {
@@ -635,6 +651,66 @@ static void pcc_time(oracle *o, int i) {
     job_assign(o, i, 0, &e, SDMD_PCC);
 }
 
+/* WallTime, TimePrediction.c:635-737, smooth cylinder: the wall is a
+ * pseudo-atom on the x axis at (0, box_y/2, box_z/2); the atom's x position
+ * and velocity are zeroed, so r is the radial vector, and the shell lookup
+ * runs on the squared distance to the wall surface,
+ * (box_y/2)^2 + r^2 - box_y*|r|.  Moving inward with a real root of the
+ * inner-limit quadratic leaves the outer shell limit (TBDCaptr, none for
+ * the outermost shell); otherwise the inner limit is met (WellBounc in the
+ * core shell, TBDEscap elsewhere).  The shell limits map back to radii as
+ * (box_y/2 - sqrt(limit))^2. */
+static void wall_time(oracle *o, int i) {
+    ev_t e;
+    init_event(&e);
+    double pi[4] = {0, 0, o->y[i], o->z[i]}, vi[4] = {0, 0, o->vy[i], o->vz[i]};
+    kin_t k;
+    for (int n = 1; n <= 3; n++) {
+        k.r[n] = pi[n] - o->wallpos[n];
+        k.v[n] = vi[n] - 0.0;
+    }
+    k.r2 = k.r[1] * k.r[1] + k.r[2] * k.r[2] + k.r[3] * k.r[3];
+    k.b = k.r[1] * k.v[1] + k.r[2] * k.v[2] + k.r[3] * k.v[3];
+    k.v2 = k.v[1] * k.v[1] + k.v[2] * k.v[2] + k.v[3] * k.v[3];
+    double by = o->box[2];
+    double dist2 = by * by * 0.25 + k.r2 - by * sqrt(k.r2);
+    fp_t f = {0, 0, 0, 0, 0};
+    int cs = shell_lookup(o, right_pair(o, o->type[i], o->wall_type, 0), K_COLLISION, -1 * k.b, dist2, &f, i, 0);
+    if (f.lopot == 0 && f.hipot == 0) {     /* inside the wall core: the reference exits */
+        o->error = 11;
+        return;
+    }
+    int s;
+    double d2;
+    if (k.b < 0 && calc_disc(&k, f.lo) > 0) {
+        if (cs == 1) return;
+        e.sub = SDMD_TBDCAPTR;
+        s = -1;
+        d2 = by * by * 0.25 + f.hi - by * sqrt(f.hi);
+        e.pot = f.hipot;
+    } else {
+        e.sub = (cs == 0) ? SDMD_WELLBOUNC : SDMD_TBDESCAP;
+        s = 1;
+        d2 = by * by * 0.25 + f.lo - by * sqrt(f.lo);
+        e.pot = f.lopot;
+    }
+    e.time = calc_time(&k, s, d2);
+    if (e.time < 0) e.time = INFINIT;
+    job_assign(o, i, 0, &e, SDMD_WALL);
+}
+
+int oracle_wall_time(oracle *o, int i, double *t, double *pot, int *sub) {
+    double t0 = o->evt[i], p0 = o->evpot[i];
+    int ty0 = o->evtype[i], s0 = o->evsub[i], pa0 = o->evp[i];
+    o->evt[i] = INFINIT;
+    o->evtype[i] = SDMD_INVD;
+    wall_time(o, i);
+    int got = o->evtype[i] == SDMD_WALL;
+    *t = o->evt[i]; *pot = o->evpot[i]; *sub = o->evsub[i];
+    o->evt[i] = t0; o->evpot[i] = p0; o->evtype[i] = ty0; o->evsub[i] = s0; o->evp[i] = pa0;
+    return got;
+}
+
 /* Predict, SGThread.c:158-211 */
 int oracle_predict(oracle *o, const int32_t *renew) {
     for (int n = 1; renew[n]; n++) {               /* ResetTarget */
@@ -651,6 +727,7 @@ int oracle_predict(oracle *o, const int32_t *renew) {
         thermostat_time(o, i);
         hb_time(o, i);
         hb_neighbor_time(o, i);
+        if (o->wall) wall_time(o, i);
     }
     return o->error ? -o->error : 0;
 }
@@ -876,6 +953,40 @@ static void thermostat_event(oracle *o, int i) {
     o->vx[i] = sp[1]; o->vy[i] = sp[2]; o->vz[i] = sp[3];
 }
 
+/* WallEvent, Events.c:423-499: impulse along the radial vector against an
+ * immovable wall, phi = (-b + s*sqrt(b^2 - 2 d^2 U / m)) / d^2 with d^2 the
+ * squared radius; WellBounc reflects, TBDCaptr/TBDEscap cross or bounce
+ * according to the energy test */
+static void wall_event(oracle *o, int i) {
+    double pi[4] = {0, 0, o->y[i], o->z[i]}, vi[4] = {0, 0, o->vy[i], o->vz[i]};
+    double r[4], v[4];
+    for (int n = 1; n <= 3; n++) {
+        r[n] = pi[n] - o->wallpos[n];
+        v[n] = vi[n] - 0.0;
+    }
+    double d2 = r[1] * r[1] + r[2] * r[2] + r[3] * r[3];
+    double b = r[1] * v[1] + r[2] * v[2] + r[3] * v[3];
+    double m = o->mass[i], U = o->evpot[i];
+    int s;
+    switch (o->evsub[i]) {
+    case SDMD_WELLBOUNC: s = -1; U = 0; break;
+    case SDMD_TBDCAPTR:
+        if (b * b - 2 * d2 * U / m <= 0) { U = 0; s = 1; }
+        else s = -1;
+        break;
+    case SDMD_TBDESCAP:
+        if (b * b - 2 * d2 * U / m <= 0) { U = 0; s = -1; }
+        else s = 1;
+        break;
+    default: o->error = 12; return;
+    }
+    double phi = (-1 * b + s * sqrt(b * b - 2 * d2 * U / m)) / d2;
+    if (isnan(phi)) { o->error = 13; return; }
+    o->vx[i] = o->vx[i] + phi * r[1];
+    o->vy[i] = o->vy[i] + phi * r[2];
+    o->vz[i] = o->vz[i] + phi * r[3];
+}
+
 /* PBCandCrossCellEvent + PBC("part") + LinkList("part"), Events.c:158-272 */
 static void pcc_event(oracle *o, int i) {
     int old[4] = {0, o->cx[i], o->cy[i], o->cz[i]};
@@ -883,7 +994,7 @@ static void pcc_event(oracle *o, int i) {
     int *ci[4] = {NULL, o->cx, o->cy, o->cz};
     for (int n = 1; n <= 3; n++) {
         if (old[n] == 0 || old[n] == o->cellnum[n] - 1) {
-            for (int m = 1; m <= 3; m++) {
+            for (int m = 1; m <= (o->wall ? 1 : 3); m++) {   /* PBC: a cylinder wraps x only */
                 if (c[m][i] >= o->box[m]) c[m][i] -= o->box[m];
                 else if (c[m][i] < 0) c[m][i] += o->box[m];
             }
@@ -923,6 +1034,7 @@ static void do_event(oracle *o, int t, int p) {   /* DoEvent, Events.c:25-114 */
     }
     case SDMD_THER: thermostat_event(o, t); break;
     case SDMD_PCC: pcc_event(o, t); break;
+    case SDMD_WALL: wall_event(o, t); break;
     default: o->error = 10; return;
     }
     if (ty >= 0 && ty < 16) o->per_type[ty]++;

@@ -16,6 +16,7 @@
  *                      HazardCheck)
  *   DoEvent/InteractionEvent/HBEvent/HBEnergyChange/HBBarrier/
  *   ThermostatEvent/PBCandCrossCellEvent/LinkList/PBC   Events.c
+ *   WallTime (smooth cylinder) TimePrediction.c:635-737, WallEvent Events.c:423-499
  * with the thermostat draws taken from the C library's srand()/rand(),
  * log() and sqrt() exactly as the reference calls them.
  */
@@ -55,6 +56,9 @@ int oracle_predict(oracle *o, const int32_t *renew);
 /* Predict over 1..N then CreateCBT (SGThread.c:47-62) */
 int oracle_bulk(oracle *o);
 int oracle_create_cbt(oracle *o);
+/* WallTime for atom i alone, leaving the state unchanged: 1 and the event
+ * (time from now, potential, subtype) when a wall event is offered, else 0 */
+int oracle_wall_time(oracle *o, int i, double *t, double *pot, int *sub);
 
 /* run `npops` iterations of the event loop; trace/hashes may be NULL.
  * hashes get (pop index, FNV-1a over positions, velocities, event times,

@@ -25,6 +25,16 @@ class PopRec(C.Structure):
                 ("r", _R * 2)]
 
 
+class RandState(C.Structure):
+    """sdmd_rand_state (sdmd_host.h): the glibc TYPE_3 rand() state."""
+    _fields_ = [("r", C.c_int32 * 34), ("f", C.c_int), ("b", C.c_int), ("calls", C.c_int64)]
+
+
+class RemdTemp(C.Structure):
+    """sdmd_remd_temp (sdmd_remd.h): struct REMDTempStr."""
+    _fields_ = [("num", C.c_int32), ("T", C.c_double)]
+
+
 class GpuOpts(C.Structure):
     _fields_ = [("mode", C.c_int32), ("block", C.c_int32), ("trace", C.c_int32),
                 ("profile", C.c_int32), ("trace_capacity", C.c_int64), ("hash_every", C.c_int64),
@@ -54,14 +64,15 @@ class GpuEvent(C.Structure):
 # every symbol declared in include/sdmd_host.h and include/sdmd_gpu.h
 HOST_SYMBOLS = ["sdmd_host_load", "sdmd_host_prepare", "sdmd_host_desc", "sdmd_host_normals",
                 "sdmd_normal_stream", "sdmd_host_energy",
-                "sdmd_rand_seed", "sdmd_rand_next", "sdmd_host_free", "sdmd_strerror"]
+                "sdmd_rand_seed", "sdmd_rand_next", "sdmd_host_free", "sdmd_strerror",
+                "sdmd_remd_rearrange"]
 GPU_SYMBOLS = ["sdmd_gpu_open", "sdmd_gpu_close", "sdmd_gpu_set_normals", "sdmd_gpu_prepare",
                "sdmd_gpu_run", "sdmd_gpu_predict", "sdmd_gpu_create_tree", "sdmd_gpu_update_tree",
                "sdmd_gpu_pop", "sdmd_gpu_advance", "sdmd_gpu_do_event", "sdmd_gpu_cell_move",
                "sdmd_gpu_download", "sdmd_gpu_get_tree", "sdmd_gpu_stats_get", "sdmd_gpu_get_trace",
                "sdmd_gpu_get_hashes", "sdmd_gpu_phase_times", "sdmd_gpu_last_kernel_ms",
                "sdmd_gpu_run_for", "sdmd_gpu_run_until", "sdmd_gpu_fast_stats_get", "sdmd_gpu_autotune",
-               "sdmd_gpu_run_graph"]
+               "sdmd_gpu_run_graph", "sdmd_gpu_run_events", "sdmd_gpu_energy", "sdmd_gpu_set_temperature"]
 
 
 def lib():
@@ -95,6 +106,10 @@ def lib():
     L.sdmd_gpu_run_until.argtypes = [P, C.c_double, I64]
     L.sdmd_gpu_autotune.argtypes = [P, C.c_double, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.sdmd_gpu_run_graph.argtypes = [P, C.c_int32, C.c_double]
+    L.sdmd_gpu_run_events.argtypes = [P, I64, C.c_double]
+    L.sdmd_gpu_energy.argtypes = [P, C.c_void_p, C.c_void_p]
+    L.sdmd_gpu_set_temperature.argtypes = [P, I, C.c_double]
+    L.sdmd_remd_rearrange.argtypes = [C.c_void_p, C.c_void_p, I, C.c_void_p, C.c_void_p]
     L.sdmd_gpu_fast_stats_get.argtypes = [P, C.c_int32, C.POINTER(FastStats)]
     L.sdmd_gpu_predict.argtypes = [P, I, C.c_void_p]
     L.sdmd_gpu_create_tree.argtypes = [P, I]

@@ -459,7 +459,11 @@ class System:
         return sum(len(c.names) for c in self.chains)
 
 
-def place_chains(chains, box, rng, min_gap=4.0, max_tries=2000):
+def place_chains(chains, box, rng, min_gap=4.0, max_tries=2000, cylinder=None):
+    """Random rotation and position for each chain with no atom closer than
+    min_gap to an atom of a placed chain (periodic distances).  With
+    cylinder=rmax every atom stays within rmax of the box's x axis line
+    (y = box_y/2, z = box_z/2), the smooth cylinder wall's axis."""
     from scipy.spatial import cKDTree
 
     box = np.asarray(box, dtype=float)
@@ -468,12 +472,25 @@ def place_chains(chains, box, rng, min_gap=4.0, max_tries=2000):
         ext = np.max(np.linalg.norm(ch.xyz, axis=1))
         if 2 * ext >= box.min():
             raise RuntimeError("chain too large for the box")
+        if cylinder is not None and ext >= cylinder:
+            raise RuntimeError("chain too large for the cylinder")
         for _ in range(max_tries):
             R = _random_rotation(rng)
             # any centre in the box: coordinates stay within (-L/2, 3L/2), which a
             # single periodic wrap (Events.c:220-227) brings back into [0, L)
             c = rng.uniform(0.0, box)
+            if cylinder is not None:
+                # centre uniform over the disc of radius rmax; the exact
+                # radial test below rejects placements that cross it
+                rr = cylinder * np.sqrt(rng.uniform())
+                ph = rng.uniform(0.0, 2 * np.pi)
+                c[1] = box[1] / 2 + rr * np.cos(ph)
+                c[2] = box[2] / 2 + rr * np.sin(ph)
             xyz = ch.xyz @ R.T + c
+            if cylinder is not None:
+                rad = np.hypot(xyz[:, 1] - box[1] / 2, xyz[:, 2] - box[2] / 2)
+                if rad.max() > cylinder:
+                    continue
             ok = True
             if placed:
                 wrapped = np.mod(xyz, box)
@@ -534,6 +551,34 @@ def ala10_extended(seed=1, box=50.0):
     ch = build_chains([["ALA"] * 10], seed, extended=True, workers=1)[0]
     ch.xyz = ch.xyz + np.array([box / 2, box / 2, box / 2])
     return System([ch], (box, box, box), "ALA10 extended chain")
+
+
+def ala10_cylinder(seed=1, box=(50.0, 24.0, 24.0)):
+    """ALA10 extended chain inside a smooth cylinder wall of radius box_y/2
+    along x: the chain's long axis is turned onto x and centred on the
+    cylinder axis (59 atoms, closest approach to the wall about 8 A)."""
+    ch = build_chains([["ALA"] * 10], seed, extended=True, workers=1)[0]
+    xyz = ch.xyz - ch.xyz.mean(axis=0)
+    _, _, vt = np.linalg.svd(xyz, full_matrices=False)
+    if np.linalg.det(vt) < 0:
+        vt[2] = -vt[2]
+    ch.xyz = xyz @ vt.T + np.array([box[0] / 2, box[1] / 2, box[2] / 2])
+    return System([ch], tuple(box), "ALA10 extended chain in a cylinder")
+
+
+def cylinder_peptides(nchains, length, seed, radius=40.0, box_x=160.0, margin=6.0, sequences=None,
+                      min_gap=4.0, workers=None, progress=None):
+    """nchains chains of uniform-random sequence (or given sequences) inside a
+    smooth cylinder wall of the given radius along x, periodic along x only:
+    box (box_x, 2*radius, 2*radius); every atom starts at least `margin` A
+    from the wall."""
+    if sequences is None:
+        sequences = random_sequences(nchains, length, seed)
+    box = (float(box_x), 2.0 * radius, 2.0 * radius)
+    chains = build_chains(sequences, seed, workers=workers, progress=progress)
+    place_chains(chains, box, np.random.default_rng([seed, 130363]), min_gap=min_gap,
+                 cylinder=radius - margin)
+    return System(chains, box, "peptides in a cylinder seed %d" % seed)
 
 
 def _build_one(args):
@@ -631,13 +676,34 @@ def batch_peptides(nsys=48, seed=4, mM=5.0, workers=None, progress=None, **kw):
     return systems
 
 
-def write_input_dir(system: System, outdir: str, gro_name="coordinate.gro", **param):
+def write_library(outdir, wall_atom="CA", wall_mass=12.0):
+    """Library_Ding for a run with a smooth wall: the shipped tables and
+    residue templates (symlinked) and a Wall.txt whose pseudo-atom type is a
+    valid atom type (ReadWall, Initialization.c:1915-1920, maps it with
+    AtomModel; the shipped file names PHO, which AtomModel does not know)."""
+    lib = os.path.join(outdir, "Library_Ding")
+    os.makedirs(lib, exist_ok=True)
+    for name in os.listdir(LIB_DIR):
+        if name == "Wall.txt":
+            continue
+        dst = os.path.join(lib, name)
+        if not os.path.lexists(dst):
+            os.symlink(os.path.join(LIB_DIR, name), dst)
+    with open(os.path.join(lib, "Wall.txt"), "w") as fh:
+        fh.write("Wall\n   ATOM %-5s %-5s %5.1f  N N\n" % (wall_atom, wall_atom, wall_mass))
+    return lib
+
+
+def write_input_dir(system: System, outdir: str, gro_name="coordinate.gro", wall_atom="CA", **param):
     """Lay out an input folder the reference and this package both read:
-    parameter.txt, the .gro file, and Library_Ding (symlinked)."""
+    parameter.txt, the .gro file, and Library_Ding (symlinked; with a wall,
+    a folder of symlinks plus a Wall.txt naming wall_atom)."""
     os.makedirs(outdir, exist_ok=True)
     write_gro(system, os.path.join(outdir, gro_name))
     write_parameter(os.path.join(outdir, "parameter.txt"), **param)
     link = os.path.join(outdir, "Library_Ding")
-    if not os.path.exists(link):
+    if param.get("wall", "no") != "no":
+        write_library(outdir, wall_atom)
+    elif not os.path.exists(link):
         os.symlink(LIB_DIR, link)
     return outdir

@@ -1240,8 +1240,6 @@ int sdmd_host_energy(const sdmd_system_desc *d, double *kinetic, double *potenti
                 double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
                 int bits = e_cmap(d, i, j);
                 if (r2 > cut2 || (bits & SDMD_BOND_CONNECT)) continue;
-                double v[3] = {d->vx[i] - d->vx[j], d->vy[i] - d->vy[j], d->vz[i] - d->vz[j]};
-                double dir = r[0] * v[0] + r[1] * v[1] + r[2] * v[2];
                 const sdmd_pair *p = NULL;
                 int a = i, b = j;
                 if (bits & SDMD_HB_CONNECT) {
@@ -1265,6 +1263,10 @@ int sdmd_host_energy(const sdmd_system_desc *d, double *kinetic, double *potenti
                 } else {
                     p = e_right_pair(d, d->type[i], d->type[j]);
                 }
+                /* the direction takes the velocities of the (possibly swapped)
+                 * pair against the unswapped r, ToolFunctions.c:722-736 */
+                double v[3] = {d->vx[a] - d->vx[b], d->vy[a] - d->vy[b], d->vz[a] - d->vz[b]};
+                double dir = r[0] * v[0] + r[1] * v[1] + r[2] * v[2];
                 double acc = e_accum(d, p, r2, dir);
                 if (acc < SDMD_INFINIT / 2) pe += acc;
             }

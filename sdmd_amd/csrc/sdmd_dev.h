@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 #define HIPV(x) ((void)(x))
+#include <limits.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -29,7 +30,7 @@ constexpr int ROWCAP = 96;                 // connection-map row staged in LDS
 enum { K_COLLISION, K_CONSTRAINT, K_HB, K_NEIGHBOR };
 enum { ERR_NONE = 0, ERR_TABLE = 1, ERR_HBMODEL = 2, ERR_SUBTYPE = 3, ERR_PHI_NAN = 4,
        ERR_DYNBITS = 5, ERR_NBLIST = 6, ERR_NBOVERFLOW = 7, ERR_CELL = 8, ERR_LINKLIST = 9,
-       ERR_EVENTTYPE = 10, ERR_CELLCAP = 11, ERR_NORMALS = 12, ERR_TRACE = 13 };
+       ERR_EVENTTYPE = 10, ERR_CELLCAP = 11, ERR_NORMALS = 12, ERR_TRACE = 13, ERR_WALL = 14 };
 
 // Device arrays are reached through pointers held in LDS-resident structs;
 // casting them to the global address space lets the compiler emit global
@@ -66,6 +67,9 @@ struct Sys {
     sdmd_pair *pair, *hbpair;
     double *sd, *se, *sacc;
     double hbpot[6];
+    // smooth cylinder wall pseudo-atom (ReadWall, Initialization.c:1907-1935)
+    int wall, wall_type;
+    double wallpos[4];
     int *node, *leaf;
     int count;
     // thermostat normal stream
@@ -161,12 +165,12 @@ static __device__ double hb_barrier(const Sys &S, int a1, int a2) {   // HBBarri
 struct FP { double lo, hi, lopot, hipot, acc; };
 
 // FindPair, ToolFunctions.c:39-224, on the flattened tables
+static __device__ __forceinline__ int shell_lookup(const Sys &S, const sdmd_pair *c, int kind, double direction,
+                                                   double d2, FP &f, int a1, int a2, int *err);
+
 static __device__ int find_pair(const Sys &S, int a1, int a2, int kind, double direction, double d2, FP &f,
                          int hbpartner, int typechange, int *err) {
     const sdmd_pair *c = nullptr;
-    int cs = -1;
-    direction = (direction < 0) ? 1 : -1;
-    double tol = direction * ZERO;
     if (kind == K_COLLISION) {
         int t1 = G(S.type)[a1];
         if (typechange == 1) t1 = type_change(t1, 1);
@@ -183,6 +187,15 @@ static __device__ int find_pair(const Sys &S, int a1, int a2, int kind, double d
         int t1 = type_change(G(S.type)[a1], 0), t2 = type_change(G(S.type)[a2], 0);
         c = &S.hbpair[(t * NT + t1) * NT + t2];
     }
+    return shell_lookup(S, c, kind, direction, d2, f, a1, a2, err);
+}
+
+// the shell search of FindPair (ToolFunctions.c:104-224) on a resolved pair
+static __device__ __forceinline__ int shell_lookup(const Sys &S, const sdmd_pair *c, int kind, double direction,
+                                                   double d2, FP &f, int a1, int a2, int *err) {
+    int cs = -1;
+    direction = (direction < 0) ? 1 : -1;
+    double tol = direction * ZERO;
     if (!c || c->dmin == 0) {
         *err = ERR_TABLE;
         f.lo = f.hi = f.acc = INFINIT;
@@ -301,6 +314,84 @@ static __device__ __forceinline__ double calc_time(const Kin &k, int s, double d
     return (-1 * k.b + s * sqrt(k.b * k.b - k.v2 * (k.rr - d2))) / k.v2;
 }
 
+// WallTime, TimePrediction.c:635-737, smooth cylinder along x: the wall is a
+// pseudo-atom at (0, box_y/2, box_z/2); the atom's x position and velocity
+// are zeroed, so r is the radial vector.  The shell lookup runs on the
+// squared distance to the wall surface, (box_y/2)^2 + r^2 - box_y*|r|, and a
+// shell limit maps back to the radius where it is met, (box_y/2 -
+// sqrt(limit))^2.  Moving inward with a real root of the inner-limit
+// quadratic leaves through the outer limit (TBDCaptr; nothing for the
+// outermost shell), otherwise the inner limit is met (WellBounc in the core
+// shell, TBDEscap elsewhere).  Returns 0 when no wall event is offered.
+struct WallEv { double t, pot; int sub; };
+
+static __device__ __forceinline__ int wall_time(const Sys &S, double y, double z, double vy, double vz, int type,
+                                                WallEv &w, int *err) {
+    Kin k;
+    k.r1 = 0.0 - S.wallpos[1];
+    k.r2 = y - S.wallpos[2];
+    k.r3 = z - S.wallpos[3];
+    const double v1 = 0.0 - 0.0, v2 = vy - 0.0, v3 = vz - 0.0;
+    k.rr = k.r1 * k.r1 + k.r2 * k.r2 + k.r3 * k.r3;
+    k.b = k.r1 * v1 + k.r2 * v2 + k.r3 * v3;
+    k.v2 = v1 * v1 + v2 * v2 + v3 * v3;
+    const double by = S.box[2];
+    const double dist2 = by * by * 0.25 + k.rr - by * sqrt(k.rr);
+    FP f;
+    const int cs = shell_lookup(S, right_pair(S, type, S.wall_type), K_COLLISION, -1 * k.b, dist2, f, 0, 0, err);
+    if (f.lopot == 0 && f.hipot == 0) {      // inside the wall's core: the reference stops here
+        *err = ERR_WALL;
+        return 0;
+    }
+    int s;
+    double d2;
+    if (k.b < 0 && calc_disc(k, f.lo) > 0) {
+        if (cs == 1) return 0;
+        w.sub = SDMD_TBDCAPTR;
+        s = -1;
+        d2 = by * by * 0.25 + f.hi - by * sqrt(f.hi);
+        w.pot = f.hipot;
+    } else {
+        w.sub = (cs == 0) ? SDMD_WELLBOUNC : SDMD_TBDESCAP;
+        s = 1;
+        d2 = by * by * 0.25 + f.lo - by * sqrt(f.lo);
+        w.pot = f.lopot;
+    }
+    w.t = calc_time(k, s, d2);
+    if (w.t < 0) w.t = INFINIT;
+    return 1;
+}
+
+// WallEvent, Events.c:423-499: impulse along the radial vector against the
+// immovable wall, phi = (-b + s*sqrt(b^2 - 2 d^2 U / m)) / d^2 with d^2 the
+// squared radius; WellBounc reflects, TBDCaptr / TBDEscap cross or bounce by
+// the energy test.  (y, z) is the atom's position at the event.
+static __device__ __forceinline__ void wall_event(const Sys &S, double y, double z, double &vx, double &vy,
+                                                  double &vz, double m, double U, int sub, int *err) {
+    const double r1 = 0.0 - S.wallpos[1], r2 = y - S.wallpos[2], r3 = z - S.wallpos[3];
+    const double v1 = 0.0 - 0.0, v2 = vy - 0.0, v3 = vz - 0.0;
+    const double d2 = r1 * r1 + r2 * r2 + r3 * r3;
+    const double b = r1 * v1 + r2 * v2 + r3 * v3;
+    int s;
+    switch (sub) {
+    case SDMD_WELLBOUNC: s = -1; U = 0; break;
+    case SDMD_TBDCAPTR:
+        if (b * b - 2 * d2 * U / m <= 0) { U = 0; s = 1; }
+        else s = -1;
+        break;
+    case SDMD_TBDESCAP:
+        if (b * b - 2 * d2 * U / m <= 0) { U = 0; s = -1; }
+        else s = 1;
+        break;
+    default: *err = ERR_SUBTYPE; return;
+    }
+    const double phi = (-1 * b + s * sqrt(b * b - 2 * d2 * U / m)) / d2;
+    if (isnan(phi)) { *err = ERR_PHI_NAN; return; }
+    vx = vx + phi * r1;
+    vy = vy + phi * r2;
+    vz = vz + phi * r3;
+}
+
 static __device__ __forceinline__ int cell_index(const Sys &S, int cx, int cy, int cz, int c) {
     int n1 = cx + c_OX[c], n2 = cy + c_OY[c], n3 = cz + c_OZ[c];
     if (n1 < 0) n1 = S.cellnum[1] - 1; else if (n1 >= S.cellnum[1]) n1 = 0;
@@ -322,6 +413,42 @@ static __device__ int cmap_get(const Sys &S, int i, int j) {
     for (int k = 0; k < SDMD_DYNSLOTS; k++)
         if (dj[k] == j) { bits |= G(S.dynb)[(size_t)i * SDMD_DYNSLOTS + k]; break; }
     return bits;
+}
+
+// One pair's term of CalPotenE (ToolFunctions.c:700-742), r = r_i - (r_j +
+// shift) with rr = |r|^2 already within the cut-off: bonded pairs add
+// nothing; a formed H-bond subtracts its barrier (HBBarrier) and reads the
+// class table; a constraint pair its own table; an H-bond neighbour pair the
+// neighbour table with the H-bonded atom's neighbour second; any other pair
+// the collision table.  The shell is looked up with the direction taken from
+// the (possibly swapped) pair's velocities against the unswapped r, and the
+// accumulated energy of that shell is added.
+static __device__ double pair_energy(const Sys &S, int i, int j, double r1, double r2, double r3, double rr,
+                                     const double *vi, const double *vj, int *err) {
+    const int bits = cmap_get(S, i, j);
+    if (bits & SDMD_BOND_CONNECT) return 0;
+    double e = 0;
+    int kind, hbp = 0;
+    bool swap = false;
+    if (bits & SDMD_HB_CONNECT) {
+        e -= hb_barrier(S, i, j);
+        kind = K_HB;
+    } else if (bits & SDMD_CONSTRAINT_CONNECT) {
+        kind = K_CONSTRAINT;
+    } else if (bits & SDMD_NEIGHBOR_CONNECT) {
+        swap = (G(S.hbnb)[G(S.hbbond)[i]] == j);
+        hbp = G(S.hbbond)[swap ? i : j];
+        kind = K_NEIGHBOR;
+    } else {
+        kind = K_COLLISION;
+    }
+    const double *va = swap ? vj : vi, *vb = swap ? vi : vj;
+    const double w1 = va[0] - vb[0], w2 = va[1] - vb[1], w3 = va[2] - vb[2];
+    const double dir = r1 * w1 + r2 * w2 + r3 * w3;
+    FP f;
+    find_pair(S, swap ? j : i, swap ? i : j, kind, dir, rr, f, hbp, 0, err);
+    if (f.acc < INFINIT / 2) e += f.acc;
+    return e;
 }
 
 static __device__ void cmap_toggle(Sys &S, int i, int j, int bit, int *err) {
@@ -541,6 +668,28 @@ static __device__ void hb_event(Sys &S, int i, int j, int nbi, int nbj, int *err
 }
 
 
+// Sum of two per-thread values over a workgroup of B threads, in a fixed
+// order (each wave by butterfly, then the waves in index order); thread 0
+// writes the two sums to out[0], out[1].
+template <int B>
+static __device__ void block_sum2(double a, double b, double *out) {
+    __shared__ double red[2 * (B / 64)];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) { red[2 * wave] = a; red[2 * wave + 1] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0, sb = 0;
+        for (int w = 0; w < B / 64; w++) { sa += red[2 * w]; sb += red[2 * w + 1]; }
+        G(out)[0] = sa;
+        G(out)[1] = sb;
+    }
+}
+
 }  // namespace sdmd
 
 // ---------------------------------------------------------------- host side
@@ -559,6 +708,7 @@ struct sdmd_gpu_ctx {
     void **allocs;
     int nalloc, capalloc;
     int *dev_renew;
+    double *dev_energy;        // 2 doubles per system (kinetic, potential)
     double last_ms;
 };
 
@@ -593,4 +743,6 @@ int sdmd_fast_stats(sdmd_gpu_ctx *c, int s, sdmd_gpu_fast_stats *out);
 int sdmd_fast_sync(sdmd_gpu_ctx *c);
 int sdmd_fast_run_graph(sdmd_gpu_ctx *c, int launches, double max_ms);
 int sdmd_fast_autotune(sdmd_gpu_ctx *c, double trial_ms, int32_t *block_out, int32_t *depth_out);
+int sdmd_fast_energy(sdmd_gpu_ctx *c, double *kinetic, double *potential);
+int sdmd_fast_set_temperature(sdmd_gpu_ctx *c, int s, double T);
 void sdmd_fast_close(sdmd_gpu_ctx *c);

@@ -1061,6 +1061,11 @@ __device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm
             }
         }
     }
+    // WallTime, TimePrediction.c:635-737
+    if (S.wall) {
+        WallEv w;
+        if (wall_time(S, A.y, A.z, A.vy, A.vz, A.type, w, err)) take(best, S, w.t, w.pot, w.sub, SDMD_WALL, 0);
+    }
     PROF_MARK(S, FPH_OTHER, pt);
     if (lane == 0) {
         G(S.evt)[i] = t + best.t;
@@ -1105,7 +1110,7 @@ __device__ void pcc_fast(FSys &F, int i, int *err) {
     int *ci[3] = {S.cx, S.cy, S.cz};
     for (int n = 0; n < 3; n++) {
         if (old[n] == 0 || old[n] == S.cellnum[n + 1] - 1) {
-            for (int m = 0; m < 3; m++) {
+            for (int m = 0; m < (S.wall ? 1 : 3); m++) {   // PBC: a cylinder wraps x only
                 if (c[m][i] >= S.box[m + 1]) c[m][i] -= S.box[m + 1];
                 else if (c[m][i] < 0) c[m][i] += S.box[m + 1];
             }
@@ -1172,6 +1177,12 @@ __device__ __forceinline__ int exec_fast(FSys &F, int a, double t, int *renew, i
     }
     case SDMD_THER: thermostat_fast(F, a); break;
     case SDMD_PCC: pcc_fast(F, a, err); break;
+    case SDMD_WALL: {
+        double vx = G(S.vx)[a], vy = G(S.vy)[a], vz = G(S.vz)[a];
+        wall_event(S, G(S.y)[a], G(S.z)[a], vx, vy, vz, G(S.mass)[a], G(S.evpot)[a], G(S.evsub)[a], err);
+        G(S.vx)[a] = vx; G(S.vy)[a] = vy; G(S.vz)[a] = vz;
+        break;
+    }
     default: *err = ERR_EVENTTYPE; return 1;
     }
     if (ty >= 0 && ty < 16) cnt_add(S.per_type[ty], 1);
@@ -1337,7 +1348,7 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
     if (tid == 0) { sh.err = S.error; sh.stop = 0; sh.drain = 0; }
     __syncthreads();
     const long long t_start = (long long)wall_clock64();
-    const long long pops0 = S.pops;
+    const long long pops0 = S.pops, frame0 = S.frame;     // npops < 0: -npops executed events
     WStats ws = {0, 0, 0, 0};
     const int cap = F.max_batch > 0 ? min(F.max_batch, FSEL) : FSEL;
     unsigned long long pt = S.prof ? wall_clock64() : 0;
@@ -1375,7 +1386,8 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
             // still pending up to the latest executed time, so that every
             // atom's local time is at most that time and every pending event
             // is later -- the state is then a snapshot at one time.
-            if (!sh.drain && (S.pops - pops0 >= npops || ((long long)wall_clock64() - t_start) > budget)) {
+            const bool spent = npops < 0 ? (S.frame - frame0 >= -npops) : (S.pops - pops0 >= npops);
+            if (!sh.drain && (spent || ((long long)wall_clock64() - t_start) > budget)) {
                 sh.drain = 1;
                 if (F.tlast < F.t_stop) F.t_stop = F.tlast;
                 sh.drain_end = S.pops + FDRAIN;
@@ -1841,6 +1853,88 @@ int sdmd_fast_download(sdmd_gpu_ctx *c, int s, sdmd_system_desc *out) {
     }
     free(t0); free(b1); free(cnt); free(mem);
     return rc;
+}
+
+// CalKinetE + CalPotenE (ToolFunctions.c:626-752) of every system at its
+// snapshot time T (the earliest pending event, the time a download reports):
+// positions are x + v (T - t0), one thread per atom i scanning the 27 cells'
+// arrays for partners j > i
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_fast_energy(FSys *systems, double *out) {
+    __shared__ FSys Fl;
+    __shared__ double tw[W];
+    fsys_load<W>(Fl, &systems[blockIdx.x]);
+    const FSys &F = Fl;
+    const Sys &S = F.s;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double tm = __builtin_inf();
+    for (int b = tid; b < F.nblk; b += 64 * W) tm = fmin(tm, G(F.b1)[b]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) tm = fmin(tm, __shfl_xor(tm, off, 64));
+    if (lane == 0) tw[wave] = tm;
+    __syncthreads();
+    double T = tw[0];
+    for (int w = 1; w < W; w++) T = fmin(T, tw[w]);
+    if (!(T < INFINIT)) T = F.tlast;
+    const int N = S.N;
+    const double cut2 = S.cutoffr * S.cutoffr;
+    double ke = 0, pe = 0;
+    int err = 0;
+    for (int i = 1 + tid; i <= N; i += 64 * W) {
+        const double dti = T - G(F.t0)[i];
+        const double vi[3] = {G(S.vx)[i], G(S.vy)[i], G(S.vz)[i]};
+        const double xi = G(S.x)[i] + vi[0] * dti, yi = G(S.y)[i] + vi[1] * dti, zi = G(S.z)[i] + vi[2] * dti;
+        ke += 0.5 * G(S.mass)[i] * (vi[0] * vi[0] + vi[1] * vi[1] + vi[2] * vi[2]);
+        const int ci[3] = {G(S.cx)[i], G(S.cy)[i], G(S.cz)[i]};
+        for (int n = 0; n < 27; n++) {
+            int cc[3] = {ci[0] + c_OX[n], ci[1] + c_OY[n], ci[2] + c_OZ[n]};
+            double sh[3] = {0, 0, 0};
+            for (int k = 0; k < 3; k++) {
+                if (cc[k] < 0) { cc[k] = S.cellnum[k + 1] - 1; sh[k] = -1 * S.box[k + 1]; }
+                else if (cc[k] >= S.cellnum[k + 1]) { cc[k] = 0; sh[k] = S.box[k + 1]; }
+            }
+            const int cell = cell_id(S, cc[0], cc[1], cc[2]);
+            const int cnt = min(G(F.ccnt)[cell], FCAP);
+            for (int q = 0; q < cnt; q++) {
+                const int j = G(F.cmem)[(size_t)cell * FCAP + q];
+                if (j <= i || j > N) continue;
+                const double dtj = T - G(F.t0)[j];
+                const double vj[3] = {G(S.vx)[j], G(S.vy)[j], G(S.vz)[j]};
+                const double xj = G(S.x)[j] + vj[0] * dtj, yj = G(S.y)[j] + vj[1] * dtj, zj = G(S.z)[j] + vj[2] * dtj;
+                const double r1 = xi - (xj + sh[0]), r2 = yi - (yj + sh[1]), r3 = zi - (zj + sh[2]);
+                const double rr = r1 * r1 + r2 * r2 + r3 * r3;
+                if (rr > cut2) continue;
+                pe += pair_energy(S, i, j, r1, r2, r3, rr, vi, vj, &err);
+            }
+        }
+    }
+    block_sum2<64 * W>(ke, pe, out + 2 * blockIdx.x);
+}
+
+int sdmd_fast_energy(sdmd_gpu_ctx *c, double *kinetic, double *potential) {
+    FastState *fs = c->fast;
+    FDISPATCH(k_fast_energy, fs->dev, c->dev_energy);
+    double *h = (double *)malloc(sizeof(double) * 2 * (size_t)c->nsys);
+    if (!h) return SDMD_ENOMEM;
+    int rc = hipGetLastError() == hipSuccess ? SDMD_OK : SDMD_EDEVICE;
+    if (!rc && (hipMemcpyAsync(h, c->dev_energy, sizeof(double) * 2 * (size_t)c->nsys, hipMemcpyDeviceToHost,
+                               c->stream) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess))
+        rc = SDMD_EDEVICE;
+    if (!rc)
+        for (int s = 0; s < c->nsys; s++) { kinetic[s] = h[2 * s]; potential[s] = h[2 * s + 1]; }
+    free(h);
+    return rc;
+}
+
+// the thermostat's target temperature of one system (REMD exchanges set it)
+int sdmd_fast_set_temperature(sdmd_gpu_ctx *c, int s, double T) {
+    FastState *fs = c->fast;
+    fs->host[s].s.T = T;
+    c->host_sys[s].T = T;
+    if (hipMemcpyAsync(&fs->dev[s].s.T, &fs->host[s].s.T, sizeof(double), hipMemcpyHostToDevice, c->stream) !=
+            hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+        return SDMD_EDEVICE;
+    return SDMD_OK;
 }
 
 int sdmd_fast_stats(sdmd_gpu_ctx *c, int s, sdmd_gpu_fast_stats *out) {

@@ -526,6 +526,11 @@ __device__ void predict_atom(Sys &S, Smem &sm, int i) {
     PROF_MARK(S, PH_HB, pt);
     if (tid == 0) {
         hb_neighbor_time(S, sm.A, &err);
+        if (S.wall) {
+            WallEv w;
+            if (wall_time(S, sm.A.y, sm.A.z, sm.A.vy, sm.A.vz, sm.A.type, w, &err))
+                job_assign_atom(sm.A, S, w.t, w.pot, w.sub, SDMD_WALL, 0);
+        }
         unstage_atom(S, sm.A);
     }
     __syncthreads();
@@ -608,7 +613,7 @@ __device__ void pcc_event(Sys &S, int i, int *err) {
     int *ci[3] = {S.cx, S.cy, S.cz};
     for (int n = 0; n < 3; n++) {
         if (old[n] == 0 || old[n] == S.cellnum[n + 1] - 1) {
-            for (int m = 0; m < 3; m++) {
+            for (int m = 0; m < (S.wall ? 1 : 3); m++) {   // PBC: a cylinder wraps x only
                 if (c[m][i] >= S.box[m + 1]) c[m][i] -= S.box[m + 1];
                 else if (c[m][i] < 0) c[m][i] += S.box[m + 1];
             }
@@ -649,6 +654,12 @@ __device__ void do_event(Sys &S, Smem &sm, int t, int p, int *err) {   // DoEven
     }
     case SDMD_THER: thermostat_event(S, sm, t); break;
     case SDMD_PCC: pcc_event(S, t, err); break;
+    case SDMD_WALL: {
+        double vx = G(S.vx)[t], vy = G(S.vy)[t], vz = G(S.vz)[t];
+        wall_event(S, G(S.y)[t], G(S.z)[t], vx, vy, vz, G(S.mass)[t], G(S.evpot)[t], G(S.evsub)[t], err);
+        G(S.vx)[t] = vx; G(S.vy)[t] = vy; G(S.vz)[t] = vz;
+        break;
+    }
     default: *err = ERR_EVENTTYPE; return;
     }
     if (ty >= 0 && ty < 16) S.per_type[ty]++;
@@ -751,11 +762,15 @@ __global__ __launch_bounds__(B) void k_run(Sys *systems, long long npops, long l
     const int N = S.N;
     unsigned long long pt = S.prof ? wall_clock64() : 0;
     const unsigned long long t_start = wall_clock64();
-    for (long long pop = 0; pop < npops; pop++) {
+    // npops < 0 asks for -npops executed events instead of pops
+    const long long maxpops = npops < 0 ? LLONG_MAX : npops, evgoal = npops < 0 ? -npops : LLONG_MAX;
+    const long long frame0 = S.frame;
+    for (long long pop = 0; pop < maxpops; pop++) {
         if (sm.err) break;
         if (tid == 0) {
             // device-time budget of a bounded run
-            if ((long long)(wall_clock64() - t_start) > budget || S.currenttime > t_end) stop = 1;
+            if ((long long)(wall_clock64() - t_start) > budget || S.currenttime > t_end || S.frame - frame0 >= evgoal)
+                stop = 1;
             // AssignJob, SGThread.c:237-259
             const int target = G(S.node)[1];
             sm.target = target;
@@ -838,6 +853,43 @@ __global__ __launch_bounds__(B) void k_run(Sys *systems, long long npops, long l
 // single operations behind the fine-grained entry points
 enum { OP_PREDICT, OP_CREATE_TREE, OP_UPDATE_TREE, OP_ADVANCE, OP_DO_EVENT, OP_CELL_MOVE };
 
+// CalKinetE + CalPotenE (ToolFunctions.c:626-752) of every system at its
+// current time, one workgroup per system, one thread per atom i scanning the
+// 27 cells (periodic wrap with position shift) for partners j > i
+template <int B>
+__global__ __launch_bounds__(B) void k_energy(Sys *systems, double *out) {
+    const Sys &S = systems[blockIdx.x];
+    const int N = S.N;
+    const double cut2 = S.cutoffr * S.cutoffr;
+    double ke = 0, pe = 0;
+    int err = 0;
+    for (int i = 1 + threadIdx.x; i <= N; i += B) {
+        const double xi = G(S.x)[i], yi = G(S.y)[i], zi = G(S.z)[i];
+        const double vi[3] = {G(S.vx)[i], G(S.vy)[i], G(S.vz)[i]};
+        ke += 0.5 * G(S.mass)[i] * (vi[0] * vi[0] + vi[1] * vi[1] + vi[2] * vi[2]);
+        const int ci[3] = {G(S.cx)[i], G(S.cy)[i], G(S.cz)[i]};
+        for (int n = 0; n < 27; n++) {
+            int cc[3] = {ci[0] + c_OX[n], ci[1] + c_OY[n], ci[2] + c_OZ[n]};
+            double sh[3] = {0, 0, 0};
+            for (int k = 0; k < 3; k++) {
+                if (cc[k] < 0) { cc[k] = S.cellnum[k + 1] - 1; sh[k] = -1 * S.box[k + 1]; }
+                else if (cc[k] >= S.cellnum[k + 1]) { cc[k] = 0; sh[k] = S.box[k + 1]; }
+            }
+            const int cell = cc[2] * S.cellnum[1] * S.cellnum[2] + cc[1] * S.cellnum[1] + cc[0] + 1;
+            int guard = 0;
+            for (int j = G(S.celllist)[N + cell]; j && ++guard <= N; j = G(S.celllist)[j]) {
+                if (j <= i) continue;
+                const double r1 = xi - (G(S.x)[j] + sh[0]), r2 = yi - (G(S.y)[j] + sh[1]), r3 = zi - (G(S.z)[j] + sh[2]);
+                const double rr = r1 * r1 + r2 * r2 + r3 * r3;
+                if (rr > cut2) continue;
+                const double vj[3] = {G(S.vx)[j], G(S.vy)[j], G(S.vz)[j]};
+                pe += pair_energy(S, i, j, r1, r2, r3, rr, vi, vj, &err);
+            }
+        }
+    }
+    block_sum2<B>(ke, pe, out + 2 * blockIdx.x);
+}
+
 template <int B>
 __global__ __launch_bounds__(B) void k_op(Sys *systems, int sys, int op, const int *renew, double dt, int a, int b) {
     __shared__ Smem sm;
@@ -900,6 +952,9 @@ static int upload_system(sdmd_gpu_ctx *c, Sys *S, const sdmd_system_desc *d, con
     S->thermoF = d->thermo_freq;
     S->thermostat = d->thermostat;
     for (int k = 0; k < 6; k++) S->hbpot[k] = d->hb_pot[k];
+    S->wall = d->wall;
+    S->wall_type = d->wall_type;
+    for (int k = 0; k < 3; k++) S->wallpos[k + 1] = d->wall_pos[k];
     S->currenttime = d->currenttime;
     S->steps_sorted = 1;
     {
@@ -977,7 +1032,6 @@ extern "C" int sdmd_gpu_open(const sdmd_system_desc *const *systems, int32_t nsy
         const sdmd_system_desc *d = systems[s];
         if (!d || d->natom < 2) return SDMD_EINVAL;
         if (d->cellnum[0] < 3 || d->cellnum[1] < 3 || d->cellnum[2] < 3) return SDMD_EINVAL;
-        if (d->wall) return SDMD_EUNSUPPORTED;
         long ncell = (long)d->cellnum[0] * d->cellnum[1] * d->cellnum[2];
         if (d->ncelllist != d->natom + ncell + 1) return SDMD_EINVAL;
     }
@@ -1045,6 +1099,10 @@ static void launch_run(sdmd_gpu_ctx *c, long long n, long long budget, double t_
     hipLaunchKernelGGL((k_run<B>), dim3(c->nsys), dim3(B), 0, c->stream, c->dev_sys, n, budget, t_end);
 }
 template <int B>
+static void launch_energy(sdmd_gpu_ctx *c) {
+    hipLaunchKernelGGL((k_energy<B>), dim3(c->nsys), dim3(B), 0, c->stream, c->dev_sys, c->dev_energy);
+}
+template <int B>
 static void launch_op(sdmd_gpu_ctx *c, int s, int op, double dt, int a, int b) {
     hipLaunchKernelGGL((k_op<B>), dim3(1), dim3(B), 0, c->stream, c->dev_sys, s, op, c->dev_renew, dt, a, b);
 }
@@ -1078,18 +1136,26 @@ extern "C" int sdmd_gpu_prepare(sdmd_gpu_ctx *c) {
 
 static int run_bounded(sdmd_gpu_ctx *c, int64_t pops, double max_ms, double t_end);
 
-extern "C" int sdmd_gpu_run(sdmd_gpu_ctx *c, int64_t pops) { return run_bounded(c, pops, 0, INFINIT); }
+extern "C" int sdmd_gpu_run(sdmd_gpu_ctx *c, int64_t pops) {
+    return pops < 0 ? SDMD_EINVAL : run_bounded(c, pops, 0, INFINIT);
+}
 
 extern "C" int sdmd_gpu_run_for(sdmd_gpu_ctx *c, int64_t pops, double max_ms) {
-    return run_bounded(c, pops, max_ms, INFINIT);
+    return pops < 0 ? SDMD_EINVAL : run_bounded(c, pops, max_ms, INFINIT);
 }
 
 extern "C" int sdmd_gpu_run_until(sdmd_gpu_ctx *c, double t_end, int64_t max_pops) {
-    return run_bounded(c, max_pops, 0, t_end);
+    return max_pops < 0 ? SDMD_EINVAL : run_bounded(c, max_pops, 0, t_end);
 }
 
+extern "C" int sdmd_gpu_run_events(sdmd_gpu_ctx *c, int64_t events, double max_ms) {
+    if (!c || events <= 0) return SDMD_EINVAL;
+    return run_bounded(c, -events, max_ms, INFINIT);
+}
+
+// pops >= 0: pops per system; pops < 0: -pops executed events per system
 static int run_bounded(sdmd_gpu_ctx *c, int64_t pops, double max_ms, double t_end) {
-    if (!c || pops < 0) return SDMD_EINVAL;
+    if (!c) return SDMD_EINVAL;
     if (c->mode == SDMD_MODE_FAST) return sdmd_fast_run(c, pops, max_ms, t_end);
     long long budget = 1LL << 62;
     if (max_ms > 0) {
@@ -1270,6 +1336,39 @@ extern "C" int sdmd_gpu_last_kernel_ms(sdmd_gpu_ctx *c, double *ms) {
     if (!c || !ms) return SDMD_EINVAL;
     *ms = c->last_ms;
     return SDMD_OK;
+}
+
+extern "C" int sdmd_gpu_energy(sdmd_gpu_ctx *c, double *kinetic, double *potential) {
+    if (!c || !kinetic || !potential) return SDMD_EINVAL;
+    int rc;
+    if (!c->dev_energy) {
+        double *zero = (double *)calloc((size_t)2 * c->nsys, sizeof(double));
+        if (!zero) return SDMD_ENOMEM;
+        rc = dev_upload(c, &c->dev_energy, zero, (size_t)2 * c->nsys);
+        free(zero);
+        if (rc) return rc;
+    }
+    if (c->mode == SDMD_MODE_FAST) return sdmd_fast_energy(c, kinetic, potential);
+    DISPATCH(launch_energy, c);
+    double *h = (double *)malloc(sizeof(double) * 2 * (size_t)c->nsys);
+    if (!h) return SDMD_ENOMEM;
+    rc = hip_ok(hipGetLastError());
+    if (!rc) rc = hip_ok(hipMemcpyAsync(h, c->dev_energy, sizeof(double) * 2 * (size_t)c->nsys, hipMemcpyDeviceToHost,
+                                        c->stream));
+    if (!rc) rc = hip_ok(hipStreamSynchronize(c->stream));
+    if (!rc)
+        for (int s = 0; s < c->nsys; s++) { kinetic[s] = h[2 * s]; potential[s] = h[2 * s + 1]; }
+    free(h);
+    return rc;
+}
+
+extern "C" int sdmd_gpu_set_temperature(sdmd_gpu_ctx *c, int32_t s, double T) {
+    if (!c || s < 0 || s >= c->nsys || !(T > 0)) return SDMD_EINVAL;
+    if (c->mode == SDMD_MODE_FAST) return sdmd_fast_set_temperature(c, s, T);
+    c->host_sys[s].T = T;
+    int rc = hip_ok(hipMemcpyAsync(&c->dev_sys[s].T, &c->host_sys[s].T, sizeof(double), hipMemcpyHostToDevice,
+                                   c->stream));
+    return rc ? rc : hip_ok(hipStreamSynchronize(c->stream));
 }
 
 extern "C" int sdmd_gpu_fast_stats_get(sdmd_gpu_ctx *c, int32_t s, sdmd_gpu_fast_stats *out) {

@@ -56,15 +56,21 @@ class System:
         return cls(h, arrays)
 
     @classmethod
-    def from_text(cls, gro_text, parameter_text, seed=1575263273, **kw):
-        """Load from in-memory .gro and parameter.txt contents."""
-        from .builder import LIB_DIR
+    def from_text(cls, gro_text, parameter_text, seed=1575263273, wall_text=None, **kw):
+        """Load from in-memory .gro and parameter.txt contents (and Wall.txt
+        for a run with a smooth wall)."""
+        from .builder import LIB_DIR, write_library
         d = tempfile.mkdtemp(prefix="sdmd_in_")
         with open(os.path.join(d, "coordinate.gro"), "w") as fh:
             fh.write(gro_text)
         with open(os.path.join(d, "parameter.txt"), "w") as fh:
             fh.write(parameter_text)
-        os.symlink(LIB_DIR, os.path.join(d, "Library_Ding"))
+        if wall_text:
+            lib = write_library(d)
+            with open(os.path.join(lib, "Wall.txt"), "w") as fh:
+                fh.write(wall_text)
+        else:
+            os.symlink(LIB_DIR, os.path.join(d, "Library_Ding"))
         return cls.load(d, "coordinate.gro", seed=seed, **kw)
 
     @property
@@ -150,6 +156,22 @@ class GpuEngine:
     def run_until(self, t_end, max_pops=1 << 40):
         """Run until the clock passes t_end (at most max_pops pops per system)."""
         check(_lib.lib().sdmd_gpu_run_until(self._h, float(t_end), int(max_pops)), "sdmd_gpu_run_until")
+
+    def run_events(self, events, max_ms=0.0):
+        """Run until every system executed `events` more events (or max_ms of
+        device time passed)."""
+        check(_lib.lib().sdmd_gpu_run_events(self._h, int(events), float(max_ms)), "sdmd_gpu_run_events")
+
+    def energy(self):
+        """(kinetic, potential) arrays over the systems, computed on the device."""
+        n = len(self.systems)
+        ke, pe = np.zeros(n), np.zeros(n)
+        check(_lib.lib().sdmd_gpu_energy(self._h, ke.ctypes.data, pe.ctypes.data), "sdmd_gpu_energy")
+        return ke, pe
+
+    def set_temperature(self, sys, temperature):
+        """Thermostat temperature of one system (reduced units)."""
+        check(_lib.lib().sdmd_gpu_set_temperature(self._h, sys, float(temperature)), "sdmd_gpu_set_temperature")
 
     def autotune(self, trial_ms=200.0):
         """Fast mode: pick the fastest workgroup shape on the live systems;

@@ -11,7 +11,7 @@ for p in (ROOT, HERE, GOLDEN):
     if p not in sys.path:
         sys.path.insert(0, p)
 
-CASES = ["ala10ext", "ala10helix", "mix4x20", "ala10nve"]
+CASES = ["ala10ext", "ala10helix", "mix4x20", "ala10nve", "ala10cyl", "cyl8x20"]
 
 
 def pytest_configure(config):
@@ -36,7 +36,8 @@ def golden_system(name):
 
     g = golden(name)
     seed = int(g["setup"]["seed"])
-    return System.from_text(str(g["input"]["gro"]), str(g["input"]["parameter"]), seed=seed), g
+    wall = str(g["input"]["wall"]) if "wall" in g["input"] else None
+    return System.from_text(str(g["input"]["gro"]), str(g["input"]["parameter"]), seed=seed, wall_text=wall), g
 
 
 def gpu_available():

@@ -44,6 +44,10 @@ CASES = {
     "ala10helix": ("helix", dict(temperature=0.5962, cutoff=9.0), 200000, 5000),
     "mix4x20": ("mix", dict(temperature=0.5962, cutoff=9.0), 100000, 3000),
     "ala10nve": ("ala10ext", dict(temperature=0.5962, cutoff=9.0, thermostat="no"), 100000, 2000),
+    "ala10cyl": ("ala10cyl", dict(temperature=0.5962, cutoff=9.0, wall="smooth", wall_type="cylinder"),
+                 100000, 5000),
+    "cyl8x20": ("cyl8x20", dict(temperature=0.5962, cutoff=7.25, wall="smooth", wall_type="cylinder"),
+                300000, 3000),
 }
 
 
@@ -56,6 +60,10 @@ def make_input(kind, d, params):
         builder.write_input_dir(builder.ala10_extended(seed=1), d, **params)
     elif kind == "mix":
         builder.write_input_dir(builder.random_peptides(4, 20, seed=7, box=80.0), d, **params)
+    elif kind == "ala10cyl":
+        builder.write_input_dir(builder.ala10_cylinder(seed=1), d, **params)
+    elif kind == "cyl8x20":
+        builder.write_input_dir(builder.cylinder_peptides(8, 20, seed=5, radius=40.0, box_x=160.0, margin=4.0), d, **params)
     elif kind == "helix":
         os.makedirs(d, exist_ok=True)
         shutil.copy(os.path.join(REF, "input", "ala10.gro"), os.path.join(d, "coordinate.gro"))
@@ -108,7 +116,13 @@ def save_case(name, dump, full, outdir, inp):
         gro = fh.read()
     with open(os.path.join(inp, "parameter.txt")) as fh:
         par = fh.read()
-    np.savez_compressed(os.path.join(outdir, name + "_input.npz"), gro=np.array(gro), parameter=np.array(par))
+    extra = {}
+    wall = os.path.join(inp, "Library_Ding", "Wall.txt")
+    if not os.path.islink(os.path.join(inp, "Library_Ding")) and os.path.exists(wall):
+        with open(wall) as fh:
+            extra["wall"] = np.array(fh.read())
+    np.savez_compressed(os.path.join(outdir, name + "_input.npz"), gro=np.array(gro), parameter=np.array(par),
+                        **extra)
 
 
 def main():

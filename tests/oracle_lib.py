@@ -55,6 +55,8 @@ def lib():
         L.oracle_currenttime.argtypes = [P]
         L.oracle_counters.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_find_pair_types.argtypes = [P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p]
+        L.oracle_wall_time.argtypes = [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_int)]
         _lib = L
     return _lib
 

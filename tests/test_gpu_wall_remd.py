@@ -1,0 +1,167 @@
+"""Smooth cylinder wall in the fast engine, the device energy, the
+per-system temperature and the REMD loop, through the C ABI.
+
+(The plain engine's wall runs are in test_gpu_parity.py: the cylinder golden
+cases replay the compiled reference's event sequence bit for bit.)"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from conftest import golden_system
+from sdmd_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+
+def host_energy(arr):
+    d = arr.desc()
+    ke, pe = C.c_double(), C.c_double()
+    assert _lib.lib().sdmd_host_energy(C.byref(d), C.byref(ke), C.byref(pe)) == 0
+    return ke.value, pe.value
+
+
+def _nve_state_near_wall(name, pops):
+    """The oracle's state after `pops` pops of a golden cylinder run, with
+    the thermostat off, as a System the engine can open."""
+    from sdmd_amd.engine import System
+
+    s, _ = golden_system(name)
+    o = oracle_lib.Oracle(s.arrays)
+    o.bulk()
+    o.run(pops, trace=False)
+    st = o.state()
+    st.scalars["thermostat"] = 0
+    return System(None, st)
+
+
+def test_fast_bulk_prediction_with_wall_matches_oracle(require_gpu):
+    from sdmd_amd.engine import GpuEngine
+
+    sysm = _nve_state_near_wall("ala10cyl", 30000)
+    eng = GpuEngine([sysm], block=256, mode="fast")
+    eng.prepare()
+    st = eng.state(0)
+    eng.close()
+    o = oracle_lib.Oracle(sysm.arrays)
+    o.bulk()
+    ref = o.state().arr
+    a = st.arr
+    T = st.scalars["currenttime"]                  # the engine's snapshot time (absolute)
+    C0 = sysm.arrays.scalars["currenttime"]          # the oracle's clock (times are relative to it)
+    for f in ("ev_partner", "ev_type", "ev_sub"):
+        bad = np.flatnonzero(a[f][1:] != ref[f][1:])
+        assert len(bad) == 0, (f, int(bad[0]) + 1)
+    rel = np.abs((a["ev_time"][1:] + T) - (ref["ev_time"][1:] + C0)) / ref["ev_time"][1:]
+    assert rel.max() < 1e-9
+    # a run this close to the wall: some atoms' next event is a wall event
+    assert np.count_nonzero(ref["ev_type"][1:] == 5) > 0
+
+
+@pytest.mark.parametrize("name", ["ala10cyl", "cyl8x20"])
+def test_fast_wall_run_screens_agree_with_fp64(require_gpu, name):
+    from sdmd_amd.engine import GpuEngine
+
+    sysm, _ = golden_system(name)
+    eng = GpuEngine([sysm], block=512, mode="fast", validate=True)
+    eng.prepare()
+    eng.run(60000 if name == "ala10cyl" else 150000)
+    st, fs = eng.stats(0), eng.fast_stats(0)
+    assert st["error"] == 0
+    assert fs["agree"] == fs["validated"] and fs["validated"] > 10000
+    assert fs["causal"] == 0
+    if name == "ala10cyl":
+        assert st["per_type"][5] > 0          # wall events executed
+    # every atom stays inside the cylinder, clear of the wall's core
+    a = eng.state(0).arr
+    sc = sysm.arrays.scalars
+    R = sc["box"][1] / 2
+    r = np.hypot(a["y"][1:] - sc["wall_pos"][1], a["z"][1:] - sc["wall_pos"][2])
+    assert r.max() < R - 1.0
+    eng.close()
+
+
+@pytest.mark.parametrize("mode", ["plain", "fast"])
+def test_device_energy_matches_host(require_gpu, mode):
+    from sdmd_amd.engine import GpuEngine
+
+    s1, _ = golden_system("cyl8x20")
+    s2, _ = golden_system("mix4x20")
+    kw = dict(max_pops=20000) if mode == "plain" else {}
+    eng = GpuEngine([s1, s2], block=256, mode=mode, **kw)
+    eng.prepare()
+    for _ in range(2):
+        ke, pe = eng.energy()
+        for k in range(2):
+            hk, hp = host_energy(eng.state(k))
+            assert ke[k] == pytest.approx(hk, rel=1e-12, abs=1e-9)
+            # per-atom partial sums in another order than the host loop
+            assert pe[k] == pytest.approx(hp, rel=1e-10, abs=1e-8)
+        eng.run(8000)
+    eng.close()
+
+
+def test_run_events_counts_executed_events(require_gpu):
+    from sdmd_amd.engine import GpuEngine
+
+    s1, _ = golden_system("mix4x20")
+    s2, _ = golden_system("ala10ext")
+    for mode in ("plain", "fast"):
+        kw = dict(max_pops=40000) if mode == "plain" else {}
+        eng = GpuEngine([s1, s2], block=256, mode=mode, **kw)
+        eng.prepare()
+        f0 = [eng.stats(k)["frame"] for k in range(2)]
+        eng.run_events(5000)
+        for k in range(2):
+            d = eng.stats(k)["frame"] - f0[k]
+            if mode == "plain":
+                assert d == 5000, (mode, k, d)
+            else:
+                assert 5000 <= d < 5000 + 4096, (mode, k, d)     # plus the closing drain
+        eng.close()
+
+
+def test_set_temperature_moves_the_thermostat(require_gpu):
+    """Andersen draws at variance T/m: after raising T the instantaneous
+    temperature 2 KE / 3N follows it."""
+    from sdmd_amd.engine import GpuEngine
+
+    s, _ = golden_system("mix4x20")
+    T0 = s.arrays.scalars["temperature"]
+    eng = GpuEngine([s, s], block=256, mode="fast")
+    eng.prepare()
+    eng.set_temperature(1, 2 * T0)
+    eng.run_events(150000)
+    samples = []
+    for _ in range(10):
+        eng.run_events(20000)
+        ke, _ = eng.energy()
+        samples.append(2 * ke / (3 * s.natom))
+    t = np.mean(samples, axis=0)
+    assert abs(t[0] / T0 - 1) < 0.15, t
+    assert abs(t[1] / (2 * T0) - 1) < 0.15, t
+    with pytest.raises(_lib.SdmdError):
+        eng.set_temperature(0, -1.0)
+    eng.close()
+
+
+def test_remd_loop_exchanges_temperatures(require_gpu):
+    from sdmd_amd.remd import Remd, ladder
+
+    s, _ = golden_system("cyl8x20")
+    n = 4
+    temps = ladder(n, 280.0, 330.0)
+    r = Remd([s] * n, temps, seed=3, interval=4000, block=256)
+    r.prepare()
+    for _ in range(6):
+        pe = r.cycle()
+        assert np.all(np.isfinite(pe))
+    T, nums = r.temperatures()
+    assert sorted(nums.tolist()) == list(range(1, n + 1))
+    assert np.allclose(np.sort(T), temps)
+    for k in range(n):                     # each system runs at the temperature it holds
+        assert r.eng.stats(k)["error"] == 0
+    assert r.counts[0::2].sum() == 6 * n
+    assert r.counts[1::2].sum() > 0        # neighbouring temperatures this close do swap
+    r.close()

@@ -167,3 +167,96 @@ def test_tree_and_cell_invariants_after_run():
                     assert (a["cellx"][j], a["celly"][j], a["cellz"][j]) == (cx, cy, cz)
                     j = cl[j]
     assert np.all(seen[1:] == 1)
+
+
+# ---------------------------------------------------------------- smooth cylinder wall
+def _wall_setup(name, pops):
+    s, _ = golden_system(name)
+    o = oracle_lib.Oracle(s.arrays)
+    o.bulk()
+    if pops:
+        o.run(pops, trace=False)
+    sc = s.arrays.scalars
+    box = np.array(sc["box"][:3], float)
+    centre = np.array(sc["wall_pos"][:3], float)
+    return s, o, box[1] / 2, centre, int(sc["wall_type"])
+
+
+@pytest.mark.parametrize("name,pops", [("ala10cyl", 30000), ("cyl8x20", 0)])
+def test_wall_times_land_on_shell_limits(name, pops):
+    """WallTime's naive quadratic in the radial plane: at the offered time the
+    atom's distance to the wall surface is the square root of the shell limit
+    the event names -- the lower limit of its shell when it moves out (or
+    does not reach the inner radius), the upper one on TBDCaptr."""
+    s, o, R, c, wtype = _wall_setup(name, pops)
+    st = o.state().arr
+    L = oracle_lib.lib()
+    checked = 0
+    for i in range(1, s.natom + 1):
+        t, pot, sub = C.c_double(), C.c_double(), C.c_int()
+        if not L.oracle_wall_time(o.h, i, C.byref(t), C.byref(pot), C.byref(sub)):
+            continue
+        if not t.value < 1e10:
+            continue
+        y, z, vy, vz = st["y"][i], st["z"][i], st["vy"][i], st["vz"][i]
+        r0 = np.hypot(y - c[1], z - c[2])
+        w0 = R - r0
+        b = (y - c[1]) * vy + (z - c[2]) * vz
+        lim = np.zeros(5)
+        L.oracle_find_pair_types(o.h, int(st["type"][i]), wtype, -b, w0 * w0, lim.ctypes.data)
+        lo, hi, lopot, hipot = lim[:4]
+        yt, zt = y + vy * t.value, z + vz * t.value
+        wt = R - np.hypot(yt - c[1], zt - c[2])
+        if sub.value == 4:                     # TBDCaptr: leaves through the upper limit
+            assert abs(wt - np.sqrt(hi)) < 1e-6, (i, wt, hi)
+            assert pot.value == hipot
+        else:
+            assert sub.value in (1, 5)         # WellBounc, TBDEscap
+            assert abs(wt - np.sqrt(lo)) < 1e-6, (i, wt, lo)
+            assert pot.value == lopot
+        assert t.value > 0
+        checked += 1
+    assert checked >= 10
+
+
+def test_wall_events_reflect_or_cross_radially():
+    """WallEvent's impulse is radial: x velocity and the tangential part in
+    the y-z plane are unchanged; a bounce negates the radial speed, a
+    crossing of a step of height U changes its square by -2U/m."""
+    s, g = golden_system("ala10cyl")
+    o = oracle_lib.Oracle(s.arrays)
+    o.bulk()
+    tr, _ = o.run(int(g["trace"]["npops"]), trace=True)
+    pops = np.flatnonzero((tr["type"] == 5) & (tr["exec"] == 1))
+    assert len(pops) >= 20
+    sc = s.arrays.scalars
+    c = np.array(sc["wall_pos"][:3], float)
+    o2 = oracle_lib.Oracle(s.arrays)
+    o2.bulk()
+    done = 0
+    kinds = set()
+    for k in pops:
+        o2.run(int(k) - done, trace=False)
+        before = o2.state().arr
+        o2.run(1, trace=False)
+        after = o2.state().arr
+        done = int(k) + 1
+        i = int(tr["target"][k])
+        m = before["mass"][i]
+        U = tr["pot"][k]
+        n = np.array([after["y"][i] - c[1], after["z"][i] - c[2]])
+        n /= np.linalg.norm(n)
+        v0 = np.array([before["vy"][i], before["vz"][i]])
+        v1 = np.array([after["vy"][i], after["vz"][i]])
+        assert after["vx"][i] == before["vx"][i]
+        t0, t1 = v0 - (v0 @ n) * n, v1 - (v1 @ n) * n
+        assert np.allclose(t0, t1, rtol=0, atol=1e-12)
+        vr0, vr1 = v0 @ n, v1 @ n
+        bounce = abs(vr1 + vr0) < 1e-12 * max(1.0, abs(vr0))
+        cross = abs(vr1 * vr1 - (vr0 * vr0 - 2 * U / m)) < 1e-9 and np.sign(vr1) == np.sign(vr0)
+        assert bounce or cross, (k, tr["sub"][k], vr0, vr1, U)
+        if tr["sub"][k] == 1:
+            assert bounce
+        kinds.add(("bounce" if bounce else "cross", int(tr["sub"][k])))
+    # this run has TBDCaptr and TBDEscap events, both crossing steps
+    assert {k[1] for k in kinds} >= {4, 5}
