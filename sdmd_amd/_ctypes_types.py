@@ -77,13 +77,23 @@ def view(ptr, n, dtype):
                                  shape=(n,))
 
 
+def _own(v):
+    """An independent copy of a ctypes array value (other values as they are)."""
+    if isinstance(v, C.Array):
+        return type(v)(*v)
+    return v
+
+
 class DescArrays:
     """Owns numpy copies of every array of a desc and can rebuild a desc
     pointing at them (so tests and the oracle can edit state freely)."""
 
     def __init__(self, desc: SystemDesc):
         d = desc
-        self.scalars = {f: getattr(d, f) for f, _ in SystemDesc._fields_
+        # fixed-size fields (cellnum, box, ...) are copied out of the struct:
+        # a ctypes array read from a field shares the struct's memory, which
+        # the desc's owner may free
+        self.scalars = {f: _own(getattr(d, f)) for f, _ in SystemDesc._fields_
                         if not isinstance(getattr(d, f), (C._Pointer,))}
         N1 = d.natom + 1
         self.arr = {}
@@ -130,7 +140,7 @@ class DescArrays:
 
     def copy(self):
         other = object.__new__(DescArrays)
-        other.scalars = dict(self.scalars)
+        other.scalars = {k: _own(v) for k, v in self.scalars.items()}
         other.arr = {k: v.copy() for k, v in self.arr.items()}
         other._desc = None
         return other

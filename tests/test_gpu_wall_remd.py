@@ -23,14 +23,21 @@ def host_energy(arr):
 
 
 def _nve_state_near_wall(name, pops):
-    """The oracle's state after `pops` pops of a golden cylinder run, with
-    the thermostat off, as a System the engine can open."""
+    """The oracle's state, thermostat off, at the first pop after `pops`
+    where some atom's pending event is a wall event, as a System the engine
+    can open."""
     from sdmd_amd.engine import System
 
     s, _ = golden_system(name)
     o = oracle_lib.Oracle(s.arrays)
     o.bulk()
-    o.run(pops, trace=False)
+    tr, _ = o.run(pops, trace=True)
+    # pops after which a re-predicted atom holds a wall event
+    cand = np.flatnonzero((tr["r"][:, 0]["type"] == 5) | (tr["r"][:, 1]["type"] == 5))
+    assert len(cand) > 0
+    o = oracle_lib.Oracle(s.arrays)
+    o.bulk()
+    o.run(int(cand[len(cand) // 2]) + 1, trace=False)
     st = o.state()
     st.scalars["thermostat"] = 0
     return System(None, st)
@@ -39,7 +46,7 @@ def _nve_state_near_wall(name, pops):
 def test_fast_bulk_prediction_with_wall_matches_oracle(require_gpu):
     from sdmd_amd.engine import GpuEngine
 
-    sysm = _nve_state_near_wall("ala10cyl", 30000)
+    sysm = _nve_state_near_wall("ala10cyl", 60000)
     eng = GpuEngine([sysm], block=256, mode="fast")
     eng.prepare()
     st = eng.state(0)
