@@ -216,6 +216,7 @@ constexpr int FCON = 16;          // constraint partners staged per predicted at
 constexpr int FCONSTEP = 8;       // steps per staged constraint
 
 struct FWave {
+    FAtom a;                     // the atom being predicted (uniform across the wave)
     int row_j[FROW], row_b[FROW];
     int dyn_j[SDMD_DYNSLOTS], dyn_b[SDMD_DYNSLOTS];
     int coff[28];
@@ -407,6 +408,10 @@ struct ScrTab {
     short rp[NT * NT];           // resolved pair per (type_i, type_j), -1 = none
     ScrPair pr[NT * NT];
     float rad[FRAD];
+    // per-lane gather map of a predicted atom's fields: lane L loads the
+    // 4-byte words gbase[L] + i*gstride[L] + goff[L] (+ gtwo[L]) of atom i
+    unsigned long long gbase[64];
+    int gstride[64], goff[64], gtwo[64];
 };
 
 // ---------------------------------------------------------------- wave helpers
@@ -472,52 +477,144 @@ struct BondPre {
     long long cj;
 };
 
-__device__ __forceinline__ void fstage(FSys &F, FWave &wm, FAtom &A, int i, double t, BondPre &bp, int &nb) {
+// 64-bit lane broadcast (the value of lane L to every lane)
+__device__ __forceinline__ long long rl64(long long v, int L) {
+    const int lo = __builtin_amdgcn_readlane((int)v, L);
+    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), L);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ int rl32(int v, int L) { return __builtin_amdgcn_readlane(v, L); }
+
+// Three memory round trips: (1) every field of atom i, one per lane, through
+// the gather map; (2) its connection row, bond entries, constraint entries
+// and the 27 cells' occupancy; (3) the bonded partners' states and the
+// constraint step radii.  Every load of a level is issued by all lanes
+// (indices clamped into range, results masked afterwards), so a level costs
+// one memory latency.
+__device__ __forceinline__ void fstage(FSys &F, const ScrTab &T, FWave &wm, FAtom &A, int i, double t, BondPre &bp,
+                                       int &nb) {
     const Sys &S = F.s;
     const int lane = threadIdx.x & 63;
     i = uni(i);
     t = uni(t);
-    A.i = i; A.t = t;
-    A.x = uni(G(S.x)[i]); A.y = uni(G(S.y)[i]); A.z = uni(G(S.z)[i]);
-    A.vx = uni(G(S.vx)[i]); A.vy = uni(G(S.vy)[i]); A.vz = uni(G(S.vz)[i]);
-    A.counter = uni(G(S.evc)[i]);
-    A.cx = uni(G(S.cx)[i]); A.cy = uni(G(S.cy)[i]); A.cz = uni(G(S.cz)[i]);
-    A.type = uni(G(S.type)[i]); A.hbbond = uni(G(S.hbbond)[i]); A.hbnb = uni(G(S.hbnb)[i]);
-    A.protein = uni(G(S.protein)[i]); A.aanum = uni(G(S.aanum)[i]); A.role = (signed char)uni((int)G(S.role)[i]);
-    A.nbstat = uni(G(S.nbstat)[i]); A.nbnum = uni(G(S.nbnum)[i]);
-    for (int k = 0; k < SDMD_NBSLOTS; k++) A.nbp[k] = uni(G(S.nbp)[(size_t)i * SDMD_NBSLOTS + k]);
-    const int o = uni(G(S.cmap_off)[i]);
-    A.nrow = uni(G(S.cmap_off)[i + 1]) - o;
-    const int bo = uni(G(S.bond_off)[i]);
-    nb = uni(G(S.bond_off)[i + 1]) - bo;
-    // bond entry and its partner (lanes 32.., at most 16 bonds)
+    // ---- level 1 (lane map of the host's gather table): 0-5 x y z vx vy
+    // vz, 6 event counter (two words); 8-16 cx cy cz type hbbond hbnb
+    // protein aanum nbstat, 17 nbnum, 18-22 neighbour slots, 24-29
+    // cmap/bond/con offsets at i and i+1, 32-39 dynamic slot atoms, 40-47
+    // their bits; the role byte is loaded by every lane
+    static_assert(SDMD_NBSLOTS == 5 && SDMD_DYNSLOTS == 8, "lane map of the level-1 gather");
+    const int *gp = (const int *)T.gbase[lane] + (size_t)i * T.gstride[lane] + T.goff[lane];
+    const int lo = G(gp)[0], hi = G(gp)[T.gtwo[lane]];
+    const int role = uni((int)G(S.role)[i]);
+    const long long w = (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+    const double ax = __longlong_as_double(rl64(w, 0)), ay = __longlong_as_double(rl64(w, 1));
+    const double az = __longlong_as_double(rl64(w, 2)), avx = __longlong_as_double(rl64(w, 3));
+    const double avy = __longlong_as_double(rl64(w, 4)), avz = __longlong_as_double(rl64(w, 5));
+    const int cx = rl32(lo, 8), cy = rl32(lo, 9), cz = rl32(lo, 10);
+    const int o = rl32(lo, 24), nrow = rl32(lo, 25) - o;
+    const int bo = rl32(lo, 26);
+    nb = rl32(lo, 27) - bo;
+    const int co = rl32(lo, 28), ncon = rl32(lo, 29) - co;
+    if (lane == 0) {
+        A.i = i; A.t = t;
+        A.x = ax; A.y = ay; A.z = az; A.vx = avx; A.vy = avy; A.vz = avz;
+        A.counter = rl64(w, 6);
+        A.cx = cx; A.cy = cy; A.cz = cz;
+        A.type = rl32(lo, 11); A.hbbond = rl32(lo, 12); A.hbnb = rl32(lo, 13);
+        A.protein = rl32(lo, 14); A.aanum = rl32(lo, 15); A.nbstat = rl32(lo, 16); A.nbnum = rl32(lo, 17);
+        A.role = (signed char)role;
+        A.nrow = nrow;
+        A.ncon = ncon;
+    }
+    for (int k = 0; k < SDMD_NBSLOTS; k++) {
+        const int v = rl32(lo, 18 + k);
+        if (lane == 0) A.nbp[k] = v;
+    }
+    {
+        const int dj = __shfl(lo, 32 + (lane & 7), 64), db = __shfl(lo, 40 + (lane & 7), 64);
+        if (lane < SDMD_DYNSLOTS) { wm.dyn_j[lane] = dj; wm.dyn_b[lane] = db; }
+    }
+    // ---- level 2
+    int rj0 = 0, rb0 = 0, rj1 = 0, rb1 = 0;
+    if (nrow > 0) {
+        const int k0 = o + min(lane, nrow - 1), k1 = o + min(lane + 64, nrow - 1);
+        rj0 = G(S.cmap_j)[k0]; rb0 = G(S.cmap_bits)[k0];
+        rj1 = G(S.cmap_j)[k1]; rb1 = G(S.cmap_bits)[k1];
+    }
     const int q = lane - 32;
-    bp.j = 0;
-    if (q >= 0 && q < nb && q < 16) {
-        bp.j = G(S.bond_j)[bo + q];
-        bp.lo = G(S.bond_dmin)[bo + q];
-        bp.hi = G(S.bond_dmax)[bo + q];
-        const int j = bp.j;
+    const bool bq = q >= 0 && q < nb && q < 16;
+    int bj = i;
+    double blo = 0, bhi = 0;
+    if (nb > 0) {
+        const int qq = bo + min(max(q, 0), nb - 1);
+        bj = G(S.bond_j)[qq]; blo = G(S.bond_dmin)[qq]; bhi = G(S.bond_dmax)[qq];
+    }
+    double cdmin = 0, cdmax = 0;
+    int cstep0 = 0, cnstep = 0, cj = 0;
+    if (ncon > 0) {
+        const int cc = co + min(lane, ncon - 1);
+        cdmin = G(S.con)[cc].dmin;
+        cdmax = G(S.con)[cc].dmax;
+        cstep0 = G(S.con)[cc].step0;
+        cnstep = G(S.con)[cc].nstep;
+        cj = G(S.con_j)[cc];
+    }
+    const bool own_con = lane < ncon && lane < FCON;
+    int cell;
+    {
+        const int cl = lane < 27 ? lane : 0;
+        int n1 = cx + c_OX[cl], n2 = cy + c_OY[cl], n3 = cz + c_OZ[cl];
+        if (n1 < 0) n1 = S.cellnum[1] - 1; else if (n1 >= S.cellnum[1]) n1 = 0;
+        if (n2 < 0) n2 = S.cellnum[2] - 1; else if (n2 >= S.cellnum[2]) n2 = 0;
+        if (n3 < 0) n3 = S.cellnum[3] - 1; else if (n3 >= S.cellnum[3]) n3 = 0;
+        cell = cell_id(S, n1, n2, n3);
+    }
+    int cnt = G(F.ccnt)[cell];
+    if (lane >= 27) cnt = 0;
+    // ---- level 3: bonded partners' states (any valid atom on other lanes)
+    // and the constraint step radii (index clamped into the lane's list)
+    {
+        const int j = bj;
         bp.xj = G(S.x)[j]; bp.yj = G(S.y)[j]; bp.zj = G(S.z)[j];
         bp.vxj = G(S.vx)[j]; bp.vyj = G(S.vy)[j]; bp.vzj = G(S.vz)[j];
         bp.t0j = G(F.t0)[j];
         bp.cj = G(S.evc)[j];
+        bp.j = bq ? j : 0;
+        bp.lo = blo;
+        bp.hi = bhi;
     }
-    // connection row (sorted by atom): bitmasks over its span, or staged in
-    // LDS for a binary search when it spans 128 ids or more
-    A.rowlo = A.nrow > 0 ? uni(G(S.cmap_j)[o]) : 0x7fffffff;
-    A.rowhi = A.nrow > 0 ? uni(G(S.cmap_j)[o + A.nrow - 1]) : -1;
-    A.spanok = (A.nrow == 0) || (A.rowhi - A.rowlo < 128);
+    float rad[FCONSTEP];
+    unsigned int ez = 0;
+    {
+        const int cs0 = own_con ? cstep0 : 0, cn = own_con ? cnstep : 0;
+#pragma unroll
+        for (int k = 0; k < FCONSTEP; k++) {
+            const int kk = cs0 + min(k, max(cn - 1, 0));
+            const double r = G(S.sd)[kk], e = G(S.se)[kk];
+            rad[k] = (float)r;
+            ez |= ((k < cn && e == 0) ? 1u : 0u) << k;
+        }
+    }
+    // ---- connection row: bitmasks over its span, or staged in LDS for a
+    // binary search when it spans 128 ids or more
+    const int rowlo = nrow > 0 ? rl32(rj0, 0) : 0x7fffffff;
+    const int rowhi = nrow <= 0 ? -1 : (nrow <= 64 ? rl32(rj0, (nrow - 1) & 63)
+                                         : nrow <= 128 ? rl32(rj1, (nrow - 65) & 63)
+                                                       : uni(G(S.cmap_j)[o + nrow - 1]));
+    const int spanok = (nrow == 0) || (rowhi - rowlo < 128);
     {
         unsigned long long mb0 = 0, mb1 = 0, mc0 = 0, mc1 = 0;
-        for (int k = lane; k < A.nrow; k += 64) {
-            const int j = G(S.cmap_j)[o + k], b = G(S.cmap_bits)[o + k];
-            if (A.spanok) {
-                const int off = j - A.rowlo;
+        for (int k = lane; k < nrow; k += 64) {
+            int j, b;
+            if (k < 64) { j = rj0; b = rb0; }
+            else if (k < 128) { j = rj1; b = rb1; }
+            else { j = G(S.cmap_j)[o + k]; b = G(S.cmap_bits)[o + k]; }
+            if (spanok) {
+                const int off = j - rowlo;
                 const unsigned long long bit = 1ull << (off & 63);
                 if (b & SDMD_BOND_CONNECT) { if (off >> 6) mb1 |= bit; else mb0 |= bit; }
                 if (b & SDMD_CONSTRAINT_CONNECT) { if (off >> 6) mc1 |= bit; else mc0 |= bit; }
-            } else if (A.nrow <= FROW) {
+            } else if (nrow <= FROW) {
                 wm.row_j[k] = j;
                 wm.row_b[k] = b;
             }
@@ -527,54 +624,44 @@ __device__ __forceinline__ void fstage(FSys &F, FWave &wm, FAtom &A, int i, doub
             mb0 |= __shfl_xor(mb0, off, 64); mb1 |= __shfl_xor(mb1, off, 64);
             mc0 |= __shfl_xor(mc0, off, 64); mc1 |= __shfl_xor(mc1, off, 64);
         }
-        A.mb[0] = (unsigned long long)uni((long long)mb0); A.mb[1] = (unsigned long long)uni((long long)mb1);
-        A.mc[0] = (unsigned long long)uni((long long)mc0); A.mc[1] = (unsigned long long)uni((long long)mc1);
+        if (lane == 0) {
+            A.rowlo = rowlo; A.rowhi = rowhi; A.spanok = spanok;
+            A.mb[0] = mb0; A.mb[1] = mb1; A.mc[0] = mc0; A.mc[1] = mc1;
+        }
     }
-    if (lane < SDMD_DYNSLOTS) {
-        wm.dyn_j[lane] = G(S.dynj)[(size_t)i * SDMD_DYNSLOTS + lane];
-        wm.dyn_b[lane] = G(S.dynb)[(size_t)i * SDMD_DYNSLOTS + lane];
-    }
-    // constraint shells (one lane each)
+    // ---- constraint shells (one lane each)
     {
-        const int co = uni(G(S.con_off)[i]);
-        A.ncon = uni(G(S.con_off)[i + 1]) - co;
         int ok = 1;
-        if (lane < A.ncon && lane < FCON) {
-            const double cdmin = G(S.con)[co + lane].dmin, cdmax = G(S.con)[co + lane].dmax;
-            const int cstep0 = G(S.con)[co + lane].step0, cnstep = G(S.con)[co + lane].nstep;
-            wm.con_j[lane] = G(S.con_j)[co + lane];
+        if (own_con) {
+            wm.con_j[lane] = cj;
             wm.con_dmin[lane] = (float)cdmin;
             wm.con_dmax[lane] = (float)cdmax;
             wm.con_nstep[lane] = cnstep;
-            unsigned int ez = 0;
             if (cnstep <= FCONSTEP) {
-                for (int q = 0; q < cnstep; q++) {
-                    wm.con_rad[lane][q] = (float)G(S.sd)[cstep0 + q];
-                    ez |= (G(S.se)[cstep0 + q] == 0 ? 1u : 0u) << q;
-                }
+#pragma unroll
+                for (int k = 0; k < FCONSTEP; k++)
+                    if (k < cnstep) wm.con_rad[lane][k] = rad[k];
             } else {
                 ok = 0;
+                ez = 0;
             }
             wm.con_ez[lane] = ez;
         }
-        A.constaged = (A.ncon <= FCON) && !__ballot(!ok);
+        const int staged = (ncon <= FCON) && !__ballot(!ok);
+        if (lane == 0) A.constaged = staged;
     }
-    int cell = 0;
-    if (lane < 27) {
-        int n1 = A.cx + c_OX[lane], n2 = A.cy + c_OY[lane], n3 = A.cz + c_OZ[lane];
-        if (n1 < 0) n1 = S.cellnum[1] - 1; else if (n1 >= S.cellnum[1]) n1 = 0;
-        if (n2 < 0) n2 = S.cellnum[2] - 1; else if (n2 >= S.cellnum[2]) n2 = 0;
-        if (n3 < 0) n3 = S.cellnum[3] - 1; else if (n3 >= S.cellnum[3]) n3 = 0;
-        cell = cell_id(S, n1, n2, n3);
-        wm.cell[lane] = cell;
-    }
+    if (lane < 27) wm.cell[lane] = cell;
     // own mirror record, as put_record writes it for a state at time t
-    A.s0.x = uni((float)(A.x - A.cx * S.cellsize[1]));
-    A.s0.y = uni((float)(A.y - A.cy * S.cellsize[2]));
-    A.s0.z = uni((float)(A.z - A.cz * S.cellsize[3]));
-    A.s0.w = uni((float)(t - F.Te));
-    A.s1.x = uni((float)A.vx); A.s1.y = uni((float)A.vy); A.s1.z = uni((float)A.vz); A.s1.w = 0.f;
-    const int cnt = (lane < 27) ? G(F.ccnt)[cell] : 0;
+    if (lane == 0) {
+        fvec4 s0, s1;
+        s0.x = (float)(ax - cx * S.cellsize[1]);
+        s0.y = (float)(ay - cy * S.cellsize[2]);
+        s0.z = (float)(az - cz * S.cellsize[3]);
+        s0.w = (float)(t - F.Te);
+        s1.x = (float)avx; s1.y = (float)avy; s1.z = (float)avz; s1.w = 0.f;
+        A.s0 = s0;
+        A.s1 = s1;
+    }
     int incl = cnt;
 #pragma unroll
     for (int off = 1; off < 32; off <<= 1) {
@@ -738,17 +825,16 @@ __device__ __forceinline__ void collision_fast(FSys &F, const ScrTab &T, FWave &
         // one pass: all record loads issued together, screens kept in registers
         fvec4 r0[FQ], r1[FQ];
         int cq[FQ];
+        // every lane loads a record for every slot (slots past the last
+        // candidate re-read the last one), so the loads issue together
 #pragma unroll
         for (int q = 0; q < FQ; q++) {
-            const int k = lane + 64 * q;
-            cq[q] = 0;
-            if (k < n) {
-                const int c = cand_cell(wm, k);
-                const size_t at = (size_t)wm.cell[c] * FCAP + (k - wm.coff[c]);
-                cq[q] = c;
-                r0[q] = G(F.rec)[2 * at];
-                r1[q] = G(F.rec)[2 * at + 1];
-            }
+            const int k = min(lane + 64 * q, n - 1);
+            const int c = cand_cell(wm, k);
+            const size_t at = (size_t)wm.cell[c] * FCAP + (k - wm.coff[c]);
+            cq[q] = c;
+            r0[q] = G(F.rec)[2 * at];
+            r1[q] = G(F.rec)[2 * at + 1];
         }
         CandS o[FQ];
         float thi = FINF;
@@ -910,10 +996,10 @@ __device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm
     const int lane = threadIdx.x & 63;
     long long &nscreen = ws.screen, &nconf = ws.conf, &nfp64 = ws.fp64;
     unsigned long long pt = S.prof ? wall_clock64() : 0;
-    FAtom A;
+    FAtom &A = wm.a;
     BondPre bp;
     int nb = 0;
-    fstage(F, wm, A, i, t, bp, nb);
+    fstage(F, T, wm, A, i, t, bp, nb);
     ws.cand += wm.coff[27] - 1;
     PROF_MARK(S, FPH_STAGE, pt);
     if (nb > 16 && lane == 0) *err = ERR_CELLCAP;      // BondTime's connect[16], TimePrediction.c:188
@@ -1700,6 +1786,31 @@ int sdmd_fast_open(sdmd_gpu_ctx *c, const sdmd_gpu_opts *o) {
                 if (d->step_e[pp.step0 + k] == 0) sp.ezero |= 1u << k;
             }
             pool += pp.nstep;
+        }
+        // the gather map of fstage's first level (device addresses of this
+        // system's arrays)
+        {
+            const Sys &Sh = F.s;
+            auto setg = [&](int L, const void *base, int stride, int off, int two) {
+                tb->gbase[L] = (unsigned long long)(uintptr_t)base;
+                tb->gstride[L] = stride;
+                tb->goff[L] = off;
+                tb->gtwo[L] = two;
+            };
+            for (int L = 0; L < 64; L++) setg(L, Sh.cx, 1, 0, 0);
+            const void *d8[7] = {Sh.x, Sh.y, Sh.z, Sh.vx, Sh.vy, Sh.vz, Sh.evc};
+            for (int L = 0; L < 7; L++) setg(L, d8[L], 2, 0, 1);
+            const void *d4[10] = {Sh.cx, Sh.cy, Sh.cz, Sh.type, Sh.hbbond, Sh.hbnb, Sh.protein, Sh.aanum, Sh.nbstat,
+                                  Sh.nbnum};
+            for (int L = 0; L < 10; L++) setg(8 + L, d4[L], 1, 0, 0);
+            for (int k = 0; k < SDMD_NBSLOTS; k++) setg(18 + k, Sh.nbp, SDMD_NBSLOTS, k, 0);
+            setg(24, Sh.cmap_off, 1, 0, 0); setg(25, Sh.cmap_off, 1, 1, 0);
+            setg(26, Sh.bond_off, 1, 0, 0); setg(27, Sh.bond_off, 1, 1, 0);
+            setg(28, Sh.con_off, 1, 0, 0); setg(29, Sh.con_off, 1, 1, 0);
+            for (int k = 0; k < SDMD_DYNSLOTS; k++) {
+                setg(32 + k, Sh.dynj, SDMD_DYNSLOTS, k, 0);
+                setg(40 + k, Sh.dynb, SDMD_DYNSLOTS, k, 0);
+            }
         }
         if (rc == SDMD_OK) {
             ScrTab *dt = nullptr;
