@@ -124,14 +124,21 @@ def diff(a, b):
     return {k: (a[k] - b[k]) for k in a if k != "errors"}
 
 
-def timed(eng, nsys, fast, steps, warmup, step_ms):
+def timed(eng, nsys, fast, steps, warmup, step_ms, graph=0):
+    """warmup + steps windows of step_ms device time; with graph > 0 a window
+    is `graph` bounded launches replayed from a captured HIP graph."""
+    def step():
+        if graph > 0:
+            eng.run_graph(graph, step_ms / graph)
+        else:
+            eng.run_for(1 << 40, step_ms)
     for _ in range(warmup):
-        eng.run_for(1 << 40, step_ms)
+        step()
     t0s = totals(eng, nsys, fast)
     t0 = time.perf_counter()
     kernel_ms = 0.0
     for _ in range(steps):
-        eng.run_for(1 << 40, step_ms)
+        step()
         kernel_ms += eng.last_kernel_ms()
     elapsed = time.perf_counter() - t0
     t1s = totals(eng, nsys, fast)
@@ -158,10 +165,14 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--step-ms", type=float, default=1000.0, help="device-time window of one step")
-    ap.add_argument("--block", type=int, default=512, help="fast engine threads per system")
-    ap.add_argument("--plain-block", type=int, default=256)
+    ap.add_argument("--block", type=int, default=512, help="fast engine threads per system (before autotuning)")
+    ap.add_argument("--autotune-ms", type=float, default=150.0,
+                    help="device time per candidate shape for the autotuner (0: keep --block)")
+    ap.add_argument("--graph", type=int, default=4, help="bounded launches per step replayed from a HIP graph "
+                                                         "(0: one launch per step)")
+    ap.add_argument("--plain-block", type=int, default=512, help="plain engine threads per system (512 measured fastest of 256/512/1024)")
     ap.add_argument("--systems", type=int, default=48)
-    ap.add_argument("--cpu-pops", type=int, default=20000)
+    ap.add_argument("--cpu-pops", type=int, default=600000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -193,21 +204,25 @@ def main():
     eng = GpuEngine(systems, block=args.block, mode="fast")
     eng.prepare()
     bulk_ms = eng.last_kernel_ms()
-    f, elapsed, kernel_ms = timed(eng, nsys, True, args.steps, args.warmup, args.step_ms)
+    depth = None
+    if args.autotune_ms > 0:
+        args.block, depth = eng.autotune(args.autotune_ms)
+    f, elapsed, kernel_ms = timed(eng, nsys, True, args.steps, args.warmup, args.step_ms, graph=args.graph)
     assert f["errors"] == {0}, f["errors"]
     eng.close()
     events = f["frame"]
     predictions = 2 * events + f["recal"] - f["onebody"]
     n_nb = f["candidates"] / max(predictions, 1)
-    bytes_launch = (predictions * (96 + n_nb * 32 + 40 + 512) + f["confirms"] * 56 + events * 2 * 104) / args.steps
-    kernel_ms_launch = kernel_ms / args.steps
+    launches = args.steps * max(args.graph, 1)
+    bytes_launch = (predictions * (96 + n_nb * 32 + 40 + 512) + f["confirms"] * 56 + events * 2 * 104) / launches
+    kernel_ms_launch = kernel_ms / launches
     achieved = bytes_launch / (kernel_ms_launch / 1e3) / 1e9
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as fh:
             per_event = json.load(fh).get("hbm_bytes_per_event")
         if per_event:
-            traffic = per_event * events / args.steps
+            traffic = per_event * events / launches
 
     line = {
         "metric": "aggregate events/s + pair-time evals/s; batched FP32-screen vs plain FP64 GPU",
@@ -226,7 +241,9 @@ def main():
                                "(%d-%d per system), 5 mM per chain, cut-off 7.25 A, 300 K Andersen; "
                                "fast engine (FP32 screen + FP64 confirm)" % (nsys, sum(natoms), min(natoms),
                                                                              max(natoms)),
-                   "systems": nsys, "natom_total": sum(natoms), "block": args.block, "step_ms": args.step_ms},
+                   "systems": nsys, "natom_total": sum(natoms), "block": args.block, "staging_depth": depth,
+                   "autotuned": args.autotune_ms > 0, "graph_launches_per_step": args.graph,
+                   "step_ms": args.step_ms},
         "pair_evals_per_s": (f["screens"] + f["pair_evals"]) / elapsed,
         "fp32_screens_per_s": f["screens"] / elapsed,
         "fp64_pair_evals_per_s": f["pair_evals"] / elapsed,

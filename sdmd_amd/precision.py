@@ -1,6 +1,6 @@
 """Precision-form report: the FP32-screen / FP64-confirm engine against FP64.
 
-    python -m sdmd_amd.precision [--events 1000000] [--checkpoints 20]
+    python -m sdmd_amd.precision [--events 1000000] [--checkpoints 20] [--only fast|plain]
 
 System: the ALA10 extended chain (59 atoms, 50 A box, cut-off 9 A, T* 0.5962)
 in NVE (no thermostat), so total energy is the conserved quantity.
@@ -24,6 +24,7 @@ import ctypes as C
 import json
 import sys
 import tempfile
+import time
 
 from . import _lib, builder
 from .engine import GpuEngine, System
@@ -85,25 +86,34 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--events", type=int, default=1000000)
     ap.add_argument("--checkpoints", type=int, default=20)
+    ap.add_argument("--only", choices=("fast", "plain"), default=None,
+                    help="run one engine (long runs are split across invocations)")
     args = ap.parse_args()
     s = nve_system()
-    e0, fser, fst, ffs = run_fast(s, args.events, args.checkpoints)
-    p0, pser, pst = run_plain(s, args.events, args.checkpoints)
-    out = {
-        "system": "ALA10 extended chain, %d atoms, NVE (thermostat off), T* 0.5962, cut-off 9 A" % s.natom,
-        "events": {"fast": fst["frame"], "plain": pst["frame"]},
-        "partner_agreement": ffs["agree"] / max(ffs["validated"], 1),
-        "validated_predictions": ffs["validated"],
-        "max_time_rel_err_vs_fp64": ffs["max_time_rel_err"],
-        "max_fp32_screen_rel_err": ffs["max_screen_rel_err"],
-        "fp32_screens": ffs["screens"], "fp64_confirms": ffs["confirms"],
-        "causal_reads": ffs["causal"],
-        "energy0": e0,
-        "fast_energy_drift": fser,
-        "plain_energy_drift": pser,
-        "fast_max_abs_drift": max(abs(x) for _, x in fser),
-        "plain_max_abs_drift": max(abs(x) for _, x in pser),
-    }
+    out = {"system": "ALA10 extended chain, %d atoms, NVE (thermostat off), T* 0.5962, cut-off 9 A" % s.natom,
+           "events": {}}
+    if args.only in (None, "fast"):
+        t0 = time.perf_counter()
+        e0, fser, fst, ffs = run_fast(s, args.events, args.checkpoints)
+        out["events"]["fast"] = fst["frame"]
+        out.update({
+            "fast_seconds": time.perf_counter() - t0,
+            "partner_agreement": ffs["agree"] / max(ffs["validated"], 1),
+            "validated_predictions": ffs["validated"],
+            "max_time_rel_err_vs_fp64": ffs["max_time_rel_err"],
+            "max_fp32_screen_rel_err": ffs["max_screen_rel_err"],
+            "fp32_screens": ffs["screens"], "fp64_confirms": ffs["confirms"],
+            "causal_reads": ffs["causal"],
+            "energy0": e0,
+            "fast_energy_drift": fser,
+            "fast_max_abs_drift": max(abs(x) for _, x in fser),
+        })
+    if args.only in (None, "plain"):
+        t0 = time.perf_counter()
+        p0, pser, pst = run_plain(s, args.events, args.checkpoints)
+        out["events"]["plain"] = pst["frame"]
+        out.update({"plain_seconds": time.perf_counter() - t0, "energy0": p0, "plain_energy_drift": pser,
+                    "plain_max_abs_drift": max(abs(x) for _, x in pser)})
     print(json.dumps(out))
 
 
