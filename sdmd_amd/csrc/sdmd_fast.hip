@@ -281,7 +281,10 @@ __device__ __forceinline__ void kin_local(const FSys &F, const FAtom &A, int j, 
 
 // one CollisionTime candidate in FP64 (TimePrediction.c:79-179), exclusions
 // already applied; returns +inf when it does not reach JobAssign
-__device__ double coli64(const FSys &F, const FAtom &A, int j, int bits, double calib, double &pot, int &sub,
+// rp: the resolved collision pair of (A.type, type of j) (RightPair,
+// ToolFunctions.c:278-292), known from the screen, so the table entry loads
+// with j's state instead of after it
+__device__ double coli64(const FSys &F, const FAtom &A, int j, int bits, int rp, double calib, double &pot, int &sub,
                          bool &reached, int *err) {
     const Sys &S = F.s;
     reached = false;
@@ -296,7 +299,7 @@ __device__ double coli64(const FSys &F, const FAtom &A, int j, int bits, double 
     if (con) {
         cs = find_pair(S, A.i, j, K_CONSTRAINT, k.b, k.rr, f, 0, 0, err);
     } else {
-        cs = find_pair(S, A.i, j, K_COLLISION, k.b, k.rr, f, 0, 0, err);
+        cs = shell_lookup(S, rp >= 0 ? &S.pair[rp] : nullptr, K_COLLISION, k.b, k.rr, f, A.i, j, err);
         if (f.lopot == 0 && f.hipot == 0) return __builtin_inf();
     }
     int s;
@@ -328,6 +331,12 @@ struct Shell {
 };
 
 // FP32 shell lookup + contact time; `rad(k)` gives step radius k
+// The screen's FP32 square roots and reciprocals use the hardware
+// instructions (about 1 ulp): every screened quantity carries an error band
+// of 1e-5 relative or more, and every decision inside a band goes to FP64.
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 template <class R>
 __device__ __forceinline__ Scr screen_shell(const Shell &c, const R &rad, bool con, float rr, float b, float v2) {
     Scr o;
@@ -351,7 +360,7 @@ __device__ __forceinline__ Scr screen_shell(const Shell &c, const R &rad, bool c
     const bool lopot0 = ks ? ((c.ezero >> (ks - 1)) & 1) : true;
     bool hipot0;
     int cs;
-    const float eb = 1e-5f * sqrtf(rr * v2) + 4e-6f * sqrtf(v2) + 1e-20f;
+    const float eb = 1e-5f * fsqrt(rr * v2) + 4e-6f * fsqrt(v2) + 1e-20f;
     if (fabsf(b) <= eb) { o.flag = true; return o; }
     if (ks < c.nstep) {
         hipot0 = (c.ezero >> ks) & 1;
@@ -376,20 +385,20 @@ __device__ __forceinline__ Scr screen_shell(const Shell &c, const R &rad, bool c
     if (b < 0 && fabsf(disc) <= ed) { o.flag = true; return o; }
     float D;
     if (b < 0 && disc > 0) {
-        D = -b + sqrtf(disc);
-        o.t = (rr - lo) / D;
-        o.e = 4.f * (band + o.t * (eb + ed / (2.f * sqrtf(disc)))) / D + 1e-4f * o.t;
+        D = -b + fsqrt(disc);
+        o.t = (rr - lo) * frcp(D);
+        o.e = 4.f * (band + o.t * (eb + ed * frcp(2.f * fsqrt(disc)))) * frcp(D) + 1e-4f * o.t;
     } else {
         if (!con && cs == 1) return o;
         const float dh = hi - rr;
         const float disc2 = b * b + v2 * dh;
         if (b > 0) {
-            D = b + sqrtf(disc2);
-            o.t = dh / D;
-            o.e = 4.f * (band + o.t * (eb + (v2 * band + 2.f * b * eb) / (2.f * sqrtf(disc2)))) / D + 1e-4f * o.t;
+            D = b + fsqrt(disc2);
+            o.t = dh * frcp(D);
+            o.e = 4.f * (band + o.t * (eb + (v2 * band + 2.f * b * eb) * frcp(2.f * fsqrt(disc2)))) * frcp(D) + 1e-4f * o.t;
         } else {
-            o.t = (-b + sqrtf(disc2)) / v2;
-            o.e = 1e-4f * o.t + 4.f * (eb + band) / sqrtf(v2 + 1e-30f);
+            o.t = (-b + fsqrt(disc2)) * frcp(v2);
+            o.e = 1e-4f * o.t + 4.f * (eb + band) * frcp(fsqrt(v2 + 1e-30f));
         }
     }
     if (!(o.t > o.e)) o.flag = true;                         // sign of t uncertain
@@ -714,6 +723,7 @@ __device__ __forceinline__ int cand_cell(const FWave &wm, int k) {
 struct CandS {
     float t, e;       // screened time and its absolute error bound (t = FINF: no event)
     int j, bits;
+    int rp;           // resolved collision pair (-1: none or a constraint pair)
     bool flag;        // a decision lies in the error band: FP64 decides
     bool live;        // passed the exclusions and the scan radius
 };
@@ -723,7 +733,7 @@ __device__ __forceinline__ void screen_at(const FSys &F, const ScrTab &T, const 
                                           const fvec4 &r1, const fvec4 &s0, const fvec4 &s1, float tA,
                                           float calib, float cband, CandS &o, int &causal) {
     const Sys &S = F.s;
-    o.t = FINF; o.e = 0; o.flag = false; o.live = false;
+    o.t = FINF; o.e = 0; o.flag = false; o.live = false; o.rp = -1;
     const int id = __float_as_int(r1.w);
     const int j = id >> 8, tj = id & 255;
     o.j = j;
@@ -771,6 +781,7 @@ __device__ __forceinline__ void screen_at(const FSys &F, const ScrTab &T, const 
         }
     } else {
         const int ri = T.rp[A.type * NT + tj];
+        o.rp = ri;
         if (ri < 0) {
             sc.flag = true; sc.t = FINF; sc.e = 0;
         } else {
@@ -814,7 +825,7 @@ __device__ __forceinline__ void collision_fast(FSys &F, const ScrTab &T, FWave &
         double pot;
         int sub;
         bool reached;
-        const double t = coli64(F, A, o.j, o.bits, calib64, pot, sub, reached, err);
+        const double t = coli64(F, A, o.j, o.bits, o.rp, calib64, pot, sub, reached, err);
         if (need) {
             my_conf++;
             if (t < mine.t) { mine.t = t; mine.pot = pot; mine.sub = sub; mine.partner = o.j; mink = k; myt32 = o.t; }
@@ -862,10 +873,10 @@ __device__ __forceinline__ void collision_fast(FSys &F, const ScrTab &T, FWave &
             const int q = __builtin_ctz(todo);
             todo &= todo - 1;
             CandS oq;
-            oq.j = o[0].j; oq.bits = o[0].bits; oq.t = o[0].t;
+            oq.j = o[0].j; oq.bits = o[0].bits; oq.t = o[0].t; oq.rp = o[0].rp;
 #pragma unroll
             for (int r = 1; r < FQ; r++)
-                if (q == r) { oq.j = o[r].j; oq.bits = o[r].bits; oq.t = o[r].t; }
+                if (q == r) { oq.j = o[r].j; oq.bits = o[r].bits; oq.t = o[r].t; oq.rp = o[r].rp; }
             confirm(lane + 64 * q, oq, (needm >> q) & 1);
         }
     } else {
