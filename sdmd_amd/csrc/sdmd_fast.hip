@@ -217,6 +217,10 @@ constexpr int FCONSTEP = 8;       // steps per staged constraint
 
 struct FWave {
     FAtom a;                     // the atom being predicted (uniform across the wave)
+    // its bonded partners' states (lanes 32..47 stage bond q = lane - 32)
+    double bxj[16], byj[16], bzj[16], bvx[16], bvy[16], bvz[16], bt0[16], blo[16], bhi[16];
+    long long bcj[16];
+    int bjj[16];
     int row_j[FROW], row_b[FROW];
     int dyn_j[SDMD_DYNSLOTS], dyn_b[SDMD_DYNSLOTS];
     int coff[28];
@@ -500,8 +504,7 @@ __device__ __forceinline__ int rl32(int v, int L) { return __builtin_amdgcn_read
 // constraint step radii.  Every load of a level is issued by all lanes
 // (indices clamped into range, results masked afterwards), so a level costs
 // one memory latency.
-__device__ __forceinline__ void fstage(FSys &F, const ScrTab &T, FWave &wm, FAtom &A, int i, double t, BondPre &bp,
-                                       int &nb) {
+__device__ __forceinline__ void fstage(FSys &F, const ScrTab &T, FWave &wm, FAtom &A, int i, double t, int &nb) {
     const Sys &S = F.s;
     const int lane = threadIdx.x & 63;
     i = uni(i);
@@ -584,13 +587,17 @@ __device__ __forceinline__ void fstage(FSys &F, const ScrTab &T, FWave &wm, FAto
     // and the constraint step radii (index clamped into the lane's list)
     {
         const int j = bj;
-        bp.xj = G(S.x)[j]; bp.yj = G(S.y)[j]; bp.zj = G(S.z)[j];
-        bp.vxj = G(S.vx)[j]; bp.vyj = G(S.vy)[j]; bp.vzj = G(S.vz)[j];
-        bp.t0j = G(F.t0)[j];
-        bp.cj = G(S.evc)[j];
-        bp.j = bq ? j : 0;
-        bp.lo = blo;
-        bp.hi = bhi;
+        const double xj = G(S.x)[j], yj = G(S.y)[j], zj = G(S.z)[j];
+        const double vxj = G(S.vx)[j], vyj = G(S.vy)[j], vzj = G(S.vz)[j];
+        const double t0j = G(F.t0)[j];
+        const long long cjj = G(S.evc)[j];
+        if (q >= 0 && q < 16) {            // into the wave's LDS slot: short register lifetimes
+            wm.bxj[q] = xj; wm.byj[q] = yj; wm.bzj[q] = zj;
+            wm.bvx[q] = vxj; wm.bvy[q] = vyj; wm.bvz[q] = vzj;
+            wm.bt0[q] = t0j; wm.bcj[q] = cjj;
+            wm.bjj[q] = bq ? j : 0;
+            wm.blo[q] = blo; wm.bhi[q] = bhi;
+        }
     }
     float rad[FCONSTEP];
     unsigned int ez = 0;
@@ -1008,9 +1015,8 @@ __device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm
     long long &nscreen = ws.screen, &nconf = ws.conf, &nfp64 = ws.fp64;
     unsigned long long pt = S.prof ? wall_clock64() : 0;
     FAtom &A = wm.a;
-    BondPre bp;
     int nb = 0;
-    fstage(F, T, wm, A, i, t, bp, nb);
+    fstage(F, T, wm, A, i, t, nb);
     ws.cand += wm.coff[27] - 1;
     PROF_MARK(S, FPH_STAGE, pt);
     if (nb > 16 && lane == 0) *err = ERR_CELLCAP;      // BondTime's connect[16], TimePrediction.c:188
@@ -1023,6 +1029,15 @@ __device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm
         Best mine = init;
         mine.t = __builtin_inf();
         const int q = lane - 32;
+        BondPre bp;
+        bp.j = 0;
+        if (q >= 0 && q < 16) {
+            bp.j = wm.bjj[q];
+            bp.xj = wm.bxj[q]; bp.yj = wm.byj[q]; bp.zj = wm.bzj[q];
+            bp.vxj = wm.bvx[q]; bp.vyj = wm.bvy[q]; bp.vzj = wm.bvz[q];
+            bp.t0j = wm.bt0[q]; bp.cj = wm.bcj[q];
+            bp.lo = wm.blo[q]; bp.hi = wm.bhi[q];
+        }
         if (bp.j > 0) {
             int sub;
             mine.t = bond64_pre(F, A, bp, sub);
@@ -1297,13 +1312,13 @@ __device__ __forceinline__ int exec_fast(FSys &F, int a, double t, int *renew, i
 // pair event seen from the other side: the earlier one's execution
 // re-predicts it, and the earlier one's independence test covers its atoms,
 // so it is skipped when ranking.
-__device__ void block_scan(const FSys &F, int b, double &t1, int &a1, double &t2, int &a2, double &t3) {
-    const int lane = threadIdx.x & 63;
-    const int a = b * 64 + lane + 1;
-    const bool in = a <= F.s.N;
-    const double t = in ? G(F.s.evt)[a] : __builtin_inf();
-    const int p = in ? G(F.s.evp)[a] : 0;
-    const int k = in ? a : 0x7fffffff;
+// the block's earliest (t1, a1) and second (t2, a2) events that are not the
+// mirror of an earlier one, and its third time t3, from each lane's atom
+// b*64 + lane + 1 (event time t, partner p; `in` false past the last atom)
+__device__ __forceinline__ void block_min3(int b, bool in, double t, int p, double &t1, int &a1, double &t2, int &a2,
+                                           double &t3) {
+    const int k = in ? b * 64 + (threadIdx.x & 63) + 1 : 0x7fffffff;
+    if (!in) p = 0;
     double m = t;
     int km = k;
     wave_argmin(m, km);
@@ -1318,6 +1333,29 @@ __device__ void block_scan(const FSys &F, int b, double &t1, int &a1, double &t2
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) w = fmin(w, __shfl_xor(w, off, 64));
     t1 = m; a1 = km; t2 = u; a2 = ku; t3 = w;
+}
+
+__device__ void block_scan(const FSys &F, int b, double &t1, int &a1, double &t2, int &a2, double &t3) {
+    const int lane = threadIdx.x & 63;
+    const int a = b * 64 + lane + 1;
+    const bool in = a <= F.s.N;
+    const double t = in ? G(F.s.evt)[a] : __builtin_inf();
+    const int p = in ? G(F.s.evp)[a] : 0;
+    block_min3(b, in, t, p, t1, a1, t2, a2, t3);
+}
+
+// block_scan of two blocks with their loads issued together (b1 < 0: none)
+__device__ __forceinline__ void block_scan2(const FSys &F, int b0, int b1, double &t1, int &a1, double &t2, int &a2,
+                                            double &t3, double &u1, int &c1, double &u2, int &c2, double &u3) {
+    const int lane = threadIdx.x & 63;
+    const int N = F.s.N;
+    const int x0 = b0 * 64 + lane + 1, x1 = b1 * 64 + lane + 1;
+    const bool in0 = x0 <= N, in1 = b1 >= 0 && x1 <= N;
+    const int r0 = in0 ? x0 : 1, r1 = in1 ? x1 : 1;
+    const double e0 = G(F.s.evt)[r0], e1 = G(F.s.evt)[r1];
+    const int p0 = G(F.s.evp)[r0], p1 = G(F.s.evp)[r1];
+    block_min3(b0, in0, in0 ? e0 : __builtin_inf(), in0 ? p0 : 0, t1, a1, t2, a2, t3);
+    if (b1 >= 0) block_min3(b1, in1, in1 ? e1 : __builtin_inf(), in1 ? p1 : 0, u1, c1, u2, c2, u3);
 }
 
 // ---------------------------------------------------------------- shared memory
@@ -1336,6 +1374,7 @@ struct FShared {
     int ren_a[2 * FSEL];
     double ren_t[2 * FSEL];
     int dirty[4 * FSEL];
+    unsigned int dmask[FMAXBLK / 32];   // blocks touched this round (bit per block)
     FWave w[FMAXW];
 };
 
@@ -1443,6 +1482,7 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
         sh.ba[b] = G(F.ba)[b]; sh.bb[b] = G(F.bb)[b];
     }
     if (tid == 0) { sh.err = S.error; sh.stop = 0; sh.drain = 0; }
+    if (tid < FMAXBLK / 32) sh.dmask[tid] = 0;
     __syncthreads();
     const long long t_start = (long long)wall_clock64();
     const long long pops0 = S.pops, frame0 = S.frame;     // npops < 0: -npops executed events
@@ -1643,23 +1683,45 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
             predict_fast<FQ>(F, tab, sh.w[wave], a, sh.ren_t[r], ws, &err);
             if (__ballot(err != 0)) break;          // uniform across the wave
         }
-        for (int r = tid; r < nren; r += 64 * W) sh.dirty[r] = (sh.ren_a[r] - 1) >> 6;
-        if (tid == 0) sh.ndirty = nren;
+        for (int r = tid; r < nren; r += 64 * W) {
+            const int b = (sh.ren_a[r] - 1) >> 6;
+            if (b >= 0 && b < nblk) atomicOr(&sh.dmask[b >> 5], 1u << (b & 31));
+        }
         if (err) atomicCAS(&sh.err, 0, err);
         __syncthreads();
         if (tid == 0) S.pops += nex;
+        // the touched blocks, each once, in ascending order (mask cleared for
+        // the next round)
+        if (wave == 0) {
+            const unsigned int m = (lane < FMAXBLK / 32) ? sh.dmask[lane] : 0u;
+            const int c = __popc(m);
+            int incl = c;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int v = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += v;
+            }
+            int o = incl - c;
+            for (unsigned int mm = m; mm; mm &= mm - 1) sh.dirty[o++] = lane * 32 + __builtin_ctz(mm);
+            if (lane < FMAXBLK / 32) sh.dmask[lane] = 0u;
+            if (lane == 63) sh.ndirty = incl;
+        }
+        __syncthreads();
         PROF_MARK(S, FPH_EXEC, pt);
-        // ---- tree: rescan the touched blocks
+        // ---- tree: rescan the touched blocks, two per wavefront at a time
+        // so their loads overlap
         const int nd = min(sh.ndirty, 4 * FSEL);
-        for (int d = wave; d < nd; d += W) {
-            const int b = sh.dirty[d];
-            bool dup = false;
-            for (int q = 0; q < d; q++) dup |= (sh.dirty[q] == b);
-            if (dup || b < 0 || b >= nblk) continue;
-            double t1, t2, t3;
-            int a1b, a2b;
-            block_scan(F, b, t1, a1b, t2, a2b, t3);
-            if (lane == 0) { sh.b1[b] = t1; sh.ba[b] = a1b; sh.b2[b] = t2; sh.bb[b] = a2b; sh.b3[b] = t3; }
+        for (int d = wave; d < nd; d += 2 * W) {
+            const int b0 = sh.dirty[d];
+            const int d1 = d + W;
+            const int b1 = d1 < nd ? sh.dirty[d1] : -1;
+            double t1, t2, t3, u1, u2, u3;
+            int a1b, a2b, c1b, c2b;
+            block_scan2(F, b0, b1, t1, a1b, t2, a2b, t3, u1, c1b, u2, c2b, u3);
+            if (lane == 0) {
+                sh.b1[b0] = t1; sh.ba[b0] = a1b; sh.b2[b0] = t2; sh.bb[b0] = a2b; sh.b3[b0] = t3;
+                if (b1 >= 0) { sh.b1[b1] = u1; sh.ba[b1] = c1b; sh.b2[b1] = u2; sh.bb[b1] = c2b; sh.b3[b1] = u3; }
+            }
         }
         __syncthreads();
         PROF_MARK(S, FPH_TREE, pt);
