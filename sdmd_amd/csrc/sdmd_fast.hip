@@ -1573,21 +1573,9 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
             } else if (lane < nsel) {
                 t = sh.sel_t[lane]; a = sh.sel_a[lane];
             }
-            // bitonic sort of (t, a) ascending over the wave
-#pragma unroll
-            for (int k = 2; k <= 64; k <<= 1) {
-#pragma unroll
-                for (int j = k >> 1; j > 0; j >>= 1) {
-                    const double to = __shfl_xor(t, j, 64);
-                    const int ao = __shfl_xor(a, j, 64);
-                    const bool up = ((lane & k) == 0);
-                    const bool lower = ((lane & j) == 0);
-                    const bool less = (to < t) || (to == t && ao < a);
-                    const bool take_o = (lower == up) ? less : !less && !(to == t && ao == a);
-                    if (take_o) { t = to; a = ao; }
-                }
-            }
-            // cells of each event's target and partner, for the independence test
+            // cells of each event's target and partner, for the independence
+            // test (loaded before the sort, which carries them along, so the
+            // loads overlap the sorting network)
             const bool valid = lane < nsel && a >= 1 && a <= S.N;
             int ca = 0, cp = 0, pcc = 0;
             if (valid) {
@@ -1595,6 +1583,21 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
                 const int p = G(S.evp)[a];
                 cp = (p > 0 && p <= S.N) ? ((G(S.cx)[p] << 20) | (G(S.cy)[p] << 10) | G(S.cz)[p]) : ca;
                 pcc = (G(S.evtype)[a] == SDMD_PCC);
+            }
+            // bitonic sort of (t, a) ascending over the wave
+#pragma unroll
+            for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    const double to = __shfl_xor(t, j, 64);
+                    const int ao = __shfl_xor(a, j, 64);
+                    const int cao = __shfl_xor(ca, j, 64), cpo = __shfl_xor(cp, j, 64), pco = __shfl_xor(pcc, j, 64);
+                    const bool up = ((lane & k) == 0);
+                    const bool lower = ((lane & j) == 0);
+                    const bool less = (to < t) || (to == t && ao < a);
+                    const bool take_o = (lower == up) ? less : !less && !(to == t && ao == a);
+                    if (take_o) { t = to; a = ao; ca = cao; cp = cpo; pcc = pco; }
+                }
             }
             if (lane < 64) { sh.sel_a[lane] = a; sh.sel_ca[lane] = ca; sh.sel_cp[lane] = cp; sh.sel_pcc[lane] = pcc; }
             if (lane == 0) { sh.nsort = nsel; sh.conflict = 0; }
