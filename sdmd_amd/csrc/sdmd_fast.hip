@@ -59,6 +59,14 @@ enum { FERR_SELECT = 20, FERR_SLOT = 21, FERR_ROW = 22, FERR_CAND = 23, FERR_NBL
 enum { FPH_SELECT, FPH_EXEC, FPH_STAGE, FPH_SCREEN, FPH_CONFIRM, FPH_OTHER, FPH_TREE, FPH_EVENT, FPH_BOND,
        FPH_SCAN, FPH_COLLECT };
 
+// step radii (squared, FP32) of one constraint entry and bit k set when
+// step k carries no energy
+struct ConRad {
+    float rad[8];
+    unsigned int ez;
+    int pad[3];
+};
+
 struct FSys {
     Sys s;
     double *t0;                 // local time of (x, v)
@@ -67,6 +75,12 @@ struct FSys {
     double *b1, *b2, *b3;       // block's earliest, second and third event times (persisted)
     int *ba, *bb;               // atoms of the earliest and second event
     unsigned int *rctr;         // per-atom draw counter
+    // fixed-stride copies of the bond lists (16 slots per atom) and FP32
+    // step radii inline with every constraint, so a prediction stages them
+    // without a dependent load
+    int *bj16;
+    double *blo16, *bhi16;
+    struct ConRad *crad;
     int ncell, nblk;
     float L32[3];
     double Te, rebase_dt;
@@ -93,9 +107,10 @@ __device__ __forceinline__ void philox10(unsigned int c[4], unsigned int k0, uns
     }
 }
 
-__device__ double fast_normal(FSys &F, int a) {
-    unsigned int c[4] = {(unsigned int)a, G(F.rctr)[a], 0x5344u, 0x4d44u};
-    G(F.rctr)[a] = c[1] + 1;
+// draw number `ctr` of atom a (the caller holds the atom's counter)
+__device__ double fast_normal_at(FSys &F, int a, unsigned int ctr) {
+    unsigned int c[4] = {(unsigned int)a, ctr, 0x5344u, 0x4d44u};
+    G(F.rctr)[a] = ctr + 1;
     philox10(c, F.key0, F.key1);
     const unsigned long long w1 = ((unsigned long long)c[0] << 32) | c[1];
     const unsigned long long w2 = ((unsigned long long)c[2] << 32) | c[3];
@@ -103,6 +118,8 @@ __device__ double fast_normal(FSys &F, int a) {
     const double u2 = ((double)(w2 >> 11) + 0.5) * 0x1.0p-53;
     return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
 }
+
+__device__ double fast_normal(FSys &F, int a) { return fast_normal_at(F, a, G(F.rctr)[a]); }
 
 // ---------------------------------------------------------------- shared counters
 // Several wavefronts update the system's counters (held in LDS) at once.
@@ -189,6 +206,7 @@ struct FAtom {
     int spanok;                  // the static row spans < 128 atom ids: bitmask lookup
     int ncon, constaged;         // constraint partners; all staged in LDS
     unsigned long long mb[2], mc[2];   // BOND / CONSTRAINT bits over rowlo .. rowlo+127
+    unsigned int rctr;           // its thermostat draw counter
     signed char role;
     int nbp[SDMD_NBSLOTS];
     fvec4 s0, s1;            // the atom's own mirror record (position at t, velocity)
@@ -498,23 +516,23 @@ __device__ __forceinline__ long long rl64(long long v, int L) {
 }
 __device__ __forceinline__ int rl32(int v, int L) { return __builtin_amdgcn_readlane(v, L); }
 
-// Three memory round trips: (1) every field of atom i, one per lane, through
-// the gather map; (2) its connection row, bond entries, constraint entries
-// and the 27 cells' occupancy; (3) the bonded partners' states and the
-// constraint step radii.  Every load of a level is issued by all lanes
-// (indices clamped into range, results masked afterwards), so a level costs
-// one memory latency.
+// Two memory round trips: (1) every field of atom i, one per lane, through
+// the gather map, with its bonded partners' ids; (2) its connection row,
+// bond limits and partners' states, constraint entries with their FP32 step
+// radii, and the 27 cells' occupancy.  Every load of a level is issued by
+// all lanes (indices clamped into range, results masked afterwards), so a
+// level costs one memory latency.
 __device__ __forceinline__ void fstage(FSys &F, const ScrTab &T, FWave &wm, FAtom &A, int i, double t, int &nb) {
     const Sys &S = F.s;
     const int lane = threadIdx.x & 63;
     i = uni(i);
     t = uni(t);
     // ---- level 1 (lane map of the host's gather table): 0-5 x y z vx vy
-    // vz, 6 event counter (two words); 8-16 cx cy cz type hbbond hbnb
+    // vz, 6 event counter (two words), 7 thermostat draw counter; 8-16 cx cy cz type hbbond hbnb
     // protein aanum nbstat, 17 nbnum, 18-22 neighbour slots, 24-29
     // cmap/bond/con offsets at i and i+1, 32-39 dynamic slot atoms, 40-47
-    // their bits; the role byte is loaded by every lane
-    static_assert(SDMD_NBSLOTS == 5 && SDMD_DYNSLOTS == 8, "lane map of the level-1 gather");
+    // their bits, 48-63 bonded partners; the role byte is loaded by every lane
+    static_assert(SDMD_NBSLOTS == 5 && SDMD_DYNSLOTS == 8 && FCONSTEP == 8, "lane map of the level-1 gather");
     const int *gp = (const int *)T.gbase[lane] + (size_t)i * T.gstride[lane] + T.goff[lane];
     const int lo = G(gp)[0], hi = G(gp)[T.gtwo[lane]];
     const int role = uni((int)G(S.role)[i]);
@@ -535,6 +553,7 @@ __device__ __forceinline__ void fstage(FSys &F, const ScrTab &T, FWave &wm, FAto
         A.type = rl32(lo, 11); A.hbbond = rl32(lo, 12); A.hbnb = rl32(lo, 13);
         A.protein = rl32(lo, 14); A.aanum = rl32(lo, 15); A.nbstat = rl32(lo, 16); A.nbnum = rl32(lo, 17);
         A.role = (signed char)role;
+        A.rctr = (unsigned int)rl32(lo, 7);
         A.nrow = nrow;
         A.ncon = ncon;
     }
@@ -553,23 +572,33 @@ __device__ __forceinline__ void fstage(FSys &F, const ScrTab &T, FWave &wm, FAto
         rj0 = G(S.cmap_j)[k0]; rb0 = G(S.cmap_bits)[k0];
         rj1 = G(S.cmap_j)[k1]; rb1 = G(S.cmap_bits)[k1];
     }
+    // bond q on lane 32 + q: its partner id came with level 1 (lane 48 + q)
     const int q = lane - 32;
     const bool bq = q >= 0 && q < nb && q < 16;
-    int bj = i;
+    const int bjs = __shfl(lo, 48 + (lane & 15), 64);
+    const int bj = (bq && bjs >= 1 && bjs <= S.N) ? bjs : i;
     double blo = 0, bhi = 0;
-    if (nb > 0) {
-        const int qq = bo + min(max(q, 0), nb - 1);
-        bj = G(S.bond_j)[qq]; blo = G(S.bond_dmin)[qq]; bhi = G(S.bond_dmax)[qq];
+    {
+        const size_t qq = (size_t)i * 16 + (lane & 15);
+        blo = G(F.blo16)[qq];
+        bhi = G(F.bhi16)[qq];
     }
     double cdmin = 0, cdmax = 0;
-    int cstep0 = 0, cnstep = 0, cj = 0;
+    int cnstep = 0, cj = 0;
+    float rad[FCONSTEP];
+    unsigned int ez = 0;
+#pragma unroll
+    for (int k = 0; k < FCONSTEP; k++) rad[k] = 0.f;
     if (ncon > 0) {
         const int cc = co + min(lane, ncon - 1);
         cdmin = G(S.con)[cc].dmin;
         cdmax = G(S.con)[cc].dmax;
-        cstep0 = G(S.con)[cc].step0;
         cnstep = G(S.con)[cc].nstep;
         cj = G(S.con_j)[cc];
+        const ConRad cr = G(F.crad)[cc];
+#pragma unroll
+        for (int k = 0; k < FCONSTEP; k++) rad[k] = cr.rad[k];
+        ez = cr.ez;
     }
     const bool own_con = lane < ncon && lane < FCON;
     int cell;
@@ -583,8 +612,7 @@ __device__ __forceinline__ void fstage(FSys &F, const ScrTab &T, FWave &wm, FAto
     }
     int cnt = G(F.ccnt)[cell];
     if (lane >= 27) cnt = 0;
-    // ---- level 3: bonded partners' states (any valid atom on other lanes)
-    // and the constraint step radii (index clamped into the lane's list)
+    // bonded partners' states (any valid atom on other lanes)
     {
         const int j = bj;
         const double xj = G(S.x)[j], yj = G(S.y)[j], zj = G(S.z)[j];
@@ -597,18 +625,6 @@ __device__ __forceinline__ void fstage(FSys &F, const ScrTab &T, FWave &wm, FAto
             wm.bt0[q] = t0j; wm.bcj[q] = cjj;
             wm.bjj[q] = bq ? j : 0;
             wm.blo[q] = blo; wm.bhi[q] = bhi;
-        }
-    }
-    float rad[FCONSTEP];
-    unsigned int ez = 0;
-    {
-        const int cs0 = own_con ? cstep0 : 0, cn = own_con ? cnstep : 0;
-#pragma unroll
-        for (int k = 0; k < FCONSTEP; k++) {
-            const int kk = cs0 + min(k, max(cn - 1, 0));
-            const double r = G(S.sd)[kk], e = G(S.se)[kk];
-            rad[k] = (float)r;
-            ez |= ((k < cn && e == 0) ? 1u : 0u) << k;
         }
     }
     // ---- connection row: bitmasks over its span, or staged in LDS for a
@@ -1078,7 +1094,7 @@ __device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm
     // ThermostatTime, TimePrediction.c:582-595
     if (S.thermostat) {
         double z = 0;
-        if (lane == 0) z = fast_normal(F, i);
+        if (lane == 0) z = fast_normal_at(F, i, A.rctr);
         z = __shfl(z, 0, 64);
         const double tt = S.thermoF * ((z < 0) ? (-z) : (z));
         take(best, S, tt, 0, SDMD_INVALID_INTER, SDMD_THER, 0);
@@ -1831,6 +1847,35 @@ int sdmd_fast_open(sdmd_gpu_ctx *c, const sdmd_gpu_opts *o) {
         if ((rc = fdev_alloc(c, &F.b3, (const double *)nullptr, (size_t)F.nblk))) return rc;
         if ((rc = fdev_alloc(c, &F.bb, (const int *)nullptr, (size_t)F.nblk))) return rc;
         if ((rc = fdev_alloc(c, &F.rctr, (const unsigned int *)nullptr, (size_t)N + 1))) return rc;
+        {
+            // bond lists at a fixed stride of 16 (BondTime's connect[16]);
+            // constraint step radii inline (CONSTRn lists, <= 8 steps staged)
+            const size_t n16 = ((size_t)N + 1) * 16;
+            int *hj = (int *)calloc(n16, sizeof(int));
+            double *hl = (double *)calloc(n16, sizeof(double)), *hh = (double *)calloc(n16, sizeof(double));
+            const int nc = d->con_off[N + 1];
+            ConRad *cr = (ConRad *)calloc(nc > 0 ? nc : 1, sizeof(ConRad));
+            if (!hj || !hl || !hh || !cr) { free(hj); free(hl); free(hh); free(cr); return SDMD_ENOMEM; }
+            for (int a = 1; a <= N; a++)
+                for (int k = d->bond_off[a], q = 0; k < d->bond_off[a + 1] && q < 16; k++, q++) {
+                    hj[(size_t)a * 16 + q] = d->bond_j[k];
+                    hl[(size_t)a * 16 + q] = d->bond_dmin[k];
+                    hh[(size_t)a * 16 + q] = d->bond_dmax[k];
+                }
+            for (int k = 0; k < nc; k++) {
+                const sdmd_pair &pc = d->con[k];
+                for (int q = 0; q < pc.nstep && q < 8; q++) {
+                    cr[k].rad[q] = (float)d->step_d[pc.step0 + q];
+                    if (d->step_e[pc.step0 + q] == 0) cr[k].ez |= 1u << q;
+                }
+            }
+            rc = fdev_alloc(c, &F.bj16, hj, n16);
+            if (rc == SDMD_OK) rc = fdev_alloc(c, &F.blo16, hl, n16);
+            if (rc == SDMD_OK) rc = fdev_alloc(c, &F.bhi16, hh, n16);
+            if (rc == SDMD_OK) rc = fdev_alloc(c, &F.crad, cr, (size_t)(nc > 0 ? nc : 1));
+            free(hj); free(hl); free(hh); free(cr);
+            if (rc) return rc;
+        }
         // FP32 screen table: RightPair (ToolFunctions.c:278-292) resolved for
         // every type pair, the pair's limits and its step radii
         ScrTab *tb = (ScrTab *)calloc(1, sizeof(ScrTab));
@@ -1874,6 +1919,8 @@ int sdmd_fast_open(sdmd_gpu_ctx *c, const sdmd_gpu_opts *o) {
                 tb->gtwo[L] = two;
             };
             for (int L = 0; L < 64; L++) setg(L, Sh.cx, 1, 0, 0);
+            setg(7, F.rctr, 1, 0, 0);
+            for (int q = 0; q < 16; q++) setg(48 + q, F.bj16, 16, q, 0);
             const void *d8[7] = {Sh.x, Sh.y, Sh.z, Sh.vx, Sh.vy, Sh.vz, Sh.evc};
             for (int L = 0; L < 7; L++) setg(L, d8[L], 2, 0, 1);
             const void *d4[10] = {Sh.cx, Sh.cy, Sh.cz, Sh.type, Sh.hbbond, Sh.hbnb, Sh.protein, Sh.aanum, Sh.nbstat,
