@@ -61,7 +61,7 @@ enum { FPH_SELECT, FPH_EXEC, FPH_STAGE, FPH_SCREEN, FPH_CONFIRM, FPH_OTHER, FPH_
 
 // step radii (squared, FP32) of one constraint entry and bit k set when
 // step k carries no energy
-struct ConRad {
+struct __attribute__((aligned(16))) ConRad {
     float rad[8];
     unsigned int ez;
     int pad[3];
@@ -595,10 +595,11 @@ __device__ __forceinline__ void fstage(FSys &F, const ScrTab &T, FWave &wm, FAto
         cdmax = G(S.con)[cc].dmax;
         cnstep = G(S.con)[cc].nstep;
         cj = G(S.con_j)[cc];
-        const ConRad cr = G(F.crad)[cc];
-#pragma unroll
-        for (int k = 0; k < FCONSTEP; k++) rad[k] = cr.rad[k];
-        ez = cr.ez;
+        const fvec4 *cv = (const fvec4 *)(F.crad + cc);
+        const fvec4 r03 = G(cv)[0], r47 = G(cv)[1];
+        rad[0] = r03.x; rad[1] = r03.y; rad[2] = r03.z; rad[3] = r03.w;
+        rad[4] = r47.x; rad[5] = r47.y; rad[6] = r47.z; rad[7] = r47.w;
+        ez = G((const unsigned int *)(F.crad + cc))[8];
     }
     const bool own_con = lane < ncon && lane < FCON;
     int cell;
