@@ -1385,7 +1385,7 @@ struct FShared {
     int sel_a[FSEL], sel_ca[FSEL], sel_cp[FSEL], sel_pcc[FSEL];
     unsigned long long conflict;
     int nsort;
-    int nsel, nex, ndirty, nren, err, stop, drain;
+    int nsel, nex, ndirty, nren, next_ren, err, stop, drain;
     long long drain_end;
     int ex_a[FSEL];
     int ren_a[2 * FSEL];
@@ -1623,6 +1623,7 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
                 if (single) F.single++;
                 sh.ndirty = 0;
                 sh.nren = 0;
+                sh.next_ren = 0;
             }
         }
         __syncthreads();
@@ -1695,10 +1696,15 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
         if (err) atomicCAS(&sh.err, 0, err);
         __syncthreads();
         PROF_MARK(S, FPH_EVENT, pt);
-        // re-prediction of every renewed atom, one wave per atom
+        // re-prediction of every renewed atom, one wave per atom; a wave takes
+        // the next atom when it finishes one (prediction times vary)
         const int nren = sh.nren;
         err = 0;
-        for (int r = wave; r < nren && sh.err == 0; r += W) {
+        for (;;) {
+            int r = 0;
+            if (lane == 0) r = atomicAdd(&sh.next_ren, 1);
+            r = __shfl(r, 0, 64);
+            if (r >= nren || sh.err != 0) break;
             const int a = sh.ren_a[r];
             predict_fast<FQ>(F, tab, sh.w[wave], a, sh.ren_t[r], ws, &err);
             if (__ballot(err != 0)) break;          // uniform across the wave
