@@ -31,10 +31,12 @@
  *                         (AssignJob, UpdateData, TimeForward, HazardCheck
  *                         215-235, DoEvent, Predict, UpdateCBT), run on the
  *                         device for a given number of pops per system
- * In SDMD_MODE_FAST only open/prepare/run/run_for/download/stats apply; the
- * single-step entry points (predict, create/update_tree, pop, advance,
- * do_event, cell_move, get_tree, traces) return SDMD_EUNSUPPORTED, as they
- * address the reference's swept, relative-time state.
+ * In SDMD_MODE_FAST the whole-loop calls apply (prepare, run, run_for,
+ * run_until, run_events, run_graph, autotune, energy, set_temperature,
+ * download, stats, fast_stats); the single-step entry points (predict,
+ * create/update_tree, pop, advance, do_event, cell_move, get_tree, traces)
+ * return SDMD_EUNSUPPORTED, as they address the reference's swept,
+ * relative-time state.
  * renew lists follow the reference's convention: renew[0] = count, atom ids
  * from renew[1], zero-terminated.
  */
@@ -59,7 +61,8 @@ enum {
 typedef struct {
     int32_t mode;          /* SDMD_MODE_PLAIN or SDMD_MODE_FAST */
     int32_t block;         /* threads per system workgroup: plain 64..1024 (power of two),
-                              fast 64, 256, 512 or 1024 (one event per wavefront); 0 = 256 */
+                              fast 64, 256, 512 or 1024 (one event per wavefront; the
+                              autotuner may also pick 768); 0 = 256 */
     int32_t trace;         /* plain: record every pop (parity checks) */
     int32_t profile;       /* plain, bit 0: accumulate per-phase device time */
     int64_t trace_capacity;
@@ -153,7 +156,7 @@ int sdmd_gpu_energy(sdmd_gpu_ctx *ctx, double *kinetic, double *potential);
  * REMD.c:100; the Andersen draws use variance T/m (ToolFunctions.c:605). */
 int sdmd_gpu_set_temperature(sdmd_gpu_ctx *ctx, int32_t sys, double temperature);
 int sdmd_gpu_fast_stats_get(sdmd_gpu_ctx *ctx, int32_t sys, sdmd_gpu_fast_stats *out);
-/* Fast mode, batching form: time every workgroup shape (4, 8 or 16
+/* Fast mode, batching form: time every workgroup shape (4, 8, 12 or 16
  * wavefronts per system, i.e. concurrent events per round and pass, and 2 or
  * 4 candidates staged per lane) for trial_ms of device time on the live
  * systems and keep the fastest; the systems advance meanwhile.  No reference

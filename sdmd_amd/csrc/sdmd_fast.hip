@@ -1968,6 +1968,7 @@ int sdmd_fast_sync(sdmd_gpu_ctx *c) { return fast_sync_from(c); }
 #define FDISPATCH(kern, ...) do { switch (fs->waves) { \
     case 4: hipLaunchKernelGGL(kern<4>, dim3(c->nsys), dim3(256), 0, c->stream, __VA_ARGS__); break; \
     case 8: hipLaunchKernelGGL(kern<8>, dim3(c->nsys), dim3(512), 0, c->stream, __VA_ARGS__); break; \
+    case 12: hipLaunchKernelGGL(kern<12>, dim3(c->nsys), dim3(768), 0, c->stream, __VA_ARGS__); break; \
     case 16: hipLaunchKernelGGL(kern<16>, dim3(c->nsys), dim3(1024), 0, c->stream, __VA_ARGS__); break; \
     default: hipLaunchKernelGGL(kern<1>, dim3(c->nsys), dim3(64), 0, c->stream, __VA_ARGS__); break; } } while (0)
 
@@ -1980,6 +1981,7 @@ static void launch_run(sdmd_gpu_ctx *c, long long pops, long long budget, double
         switch (fs->waves) {
         case 4: FRUN(4, 2); break;
         case 8: FRUN(8, 2); break;
+        case 12: FRUN(12, 2); break;
         case 16: FRUN(16, 2); break;
         default: FRUN(1, 2); break;
         }
@@ -1987,6 +1989,7 @@ static void launch_run(sdmd_gpu_ctx *c, long long pops, long long budget, double
         switch (fs->waves) {
         case 4: FRUN(4, 4); break;
         case 8: FRUN(8, 4); break;
+        case 12: FRUN(12, 4); break;
         case 16: FRUN(16, 4); break;
         default: FRUN(1, 4); break;
         }
@@ -2232,7 +2235,7 @@ int sdmd_fast_run_graph(sdmd_gpu_ctx *c, int launches, double max_ms) {
 // executes the most events per millisecond; the systems advance meanwhile.
 int sdmd_fast_autotune(sdmd_gpu_ctx *c, double trial_ms, int32_t *block_out, int32_t *depth_out) {
     FastState *fs = c->fast;
-    static const int shapes[][2] = {{4, 4}, {8, 4}, {16, 4}, {4, 2}, {8, 2}, {16, 2}};
+    static const int shapes[][2] = {{4, 4}, {8, 4}, {12, 4}, {16, 4}, {4, 2}, {8, 2}, {12, 2}, {16, 2}};
     double best = -1;
     int bw = fs->waves, bq = fs->depth;
     for (const auto &sh : shapes) {

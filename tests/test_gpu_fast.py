@@ -189,3 +189,37 @@ def test_fast_nve_energy_drift_within_reference(require_gpu):
     # same envelope
     bound = 2 * max(abs(x) for x in ref) + 1.0
     assert max(abs(x) for x in fast) <= bound, (fast, ref)
+
+
+def _run_to(system, t_end, shape=None, graph=None):
+    from sdmd_amd.engine import GpuEngine
+
+    eng = GpuEngine([system], block=512, mode="fast")
+    eng.prepare()
+    picked = eng.autotune(20.0) if shape == "auto" else None
+    if graph:
+        eng.run_graph(*graph)            # bounded launches replayed from a HIP graph, several times
+        eng.run_graph(*graph)
+    eng.run_until(t_end)
+    st, stats, fs = eng.state(0), eng.stats(0), eng.fast_stats(0)
+    eng.close()
+    return st, stats, fs, picked
+
+
+def test_fast_graph_replay_and_autotune_keep_the_trajectory(require_gpu, mid_nvt):
+    """The trajectory does not depend on how the run is cut into launches
+    (HIP-graph replays of device-time-bounded launches, each ending with its
+    closing drain) nor on the workgroup shape the autotuner picks: all reach
+    the same state at the same time, bit for bit."""
+    # late enough that the autotuner's trial launches (8 x 20 ms) end before it
+    T = 10.0
+    ref, sref, fref, _ = _run_to(mid_nvt, T)
+    assert sref["error"] == 0 and sref["frame"] > 1000
+    for kw in (dict(graph=(3, 2.0)), dict(shape="auto")):
+        st, s, fs, picked = _run_to(mid_nvt, T, **kw)
+        assert s["error"] == 0 and fs["causal"] == 0, kw
+        assert s["frame"] == sref["frame"], kw
+        for f in FIELDS:
+            assert np.array_equal(st.arr[f], ref.arr[f]), (kw, f)
+        if picked:
+            assert picked[0] in (256, 512, 768, 1024) and picked[1] in (2, 4)
