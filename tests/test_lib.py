@@ -47,3 +47,22 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
     with pytest.raises(ImportError):
         _lib.lib()
+
+
+def test_headers_are_plain_c(tmp_path):
+    """include/*.h is the boundary a C99 caller (the reference is C99)
+    compiles against: the headers must build as C with -std=c99 -pedantic."""
+    import shutil
+    import subprocess
+
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "use.c"
+    src.write_text('#include "sdmd_types.h"\n#include "sdmd_host.h"\n#include "sdmd_gpu.h"\n'
+                   '#include "sdmd_remd.h"\n'
+                   'int main(void) { sdmd_gpu_opts o = {0}; o.mode = SDMD_MODE_FAST; '
+                   'sdmd_remd_temp t = {1, 0.5}; (void)t; return o.mode == 1 ? 0 : 1; }\n')
+    r = subprocess.run([gcc, "-std=c99", "-pedantic", "-Wall", "-Werror", "-fsyntax-only",
+                        "-I", os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
