@@ -707,7 +707,7 @@ __device__ __forceinline__ void fstage(FSys &F, const ScrTab &T, FWave &wm, FAto
 }
 
 // BondTime entry (TimePrediction.c:219-293) from the staged partner state
-__device__ double bond64_pre(const FSys &F, const FAtom &A, const BondPre &bp, int &sub) {
+__device__ __forceinline__ double bond64_pre(const FSys &F, const FAtom &A, const BondPre &bp, int &sub) {
     const Sys &S = F.s;
     const int j = bp.j;
     const double dt = A.t - bp.t0j;
@@ -1385,9 +1385,11 @@ struct FShared {
     int sel_a[FSEL], sel_ca[FSEL], sel_cp[FSEL], sel_pcc[FSEL];
     unsigned long long conflict;
     int nsort;
-    int nsel, nex, ndirty, nren, next_ren, err, stop, drain;
+    int nsel, nex, ndirty, nren, nlight, next_ren, err, stop, drain;
     long long drain_end;
     int ex_a[FSEL];
+    // renewed atoms: H-bond-capable ones from the front (nren), the others
+    // from the back (nlight)
     int ren_a[2 * FSEL];
     double ren_t[2 * FSEL];
     int dirty[4 * FSEL];
@@ -1623,6 +1625,7 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
                 if (single) F.single++;
                 sh.ndirty = 0;
                 sh.nren = 0;
+                sh.nlight = 0;
                 sh.next_ren = 0;
             }
         }
@@ -1684,13 +1687,23 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
             if (e < nex) {
                 const int a = sh.ex_a[e];
                 const double t = G(S.evt)[a];
+                // H-bond-capable atoms (acceptor, donor, sulfur) scan their
+                // neighbourhood for H-bond partners: the longest predictions
+                // of a round, queued first so the waves finish together
+                const int p = G(S.evp)[a];
+                const signed char ra = G(S.role)[a], rb = G(S.role)[min(max(p, 0), S.N)];
                 int renew[2] = {0, 0};
                 const int nr = exec_fast(F, a, t, renew, &err);
                 put_record(F, renew[0]);
                 if (nr > 1) put_record(F, renew[1]);
                 cnt_max(F.tlast, t);
-                const int slot0 = atomicAdd(&sh.nren, nr);
-                for (int r = 0; r < nr; r++) { sh.ren_a[slot0 + r] = renew[r]; sh.ren_t[slot0 + r] = t; }
+                for (int r = 0; r < nr; r++) {
+                    const signed char ro = r ? rb : ra;          // renew[1] is the partner p
+                    const bool heavy = ro == 'A' || ro == 'D' || ro == 'S';
+                    const int slot = heavy ? atomicAdd(&sh.nren, 1) : 2 * FSEL - 1 - atomicAdd(&sh.nlight, 1);
+                    sh.ren_a[slot] = renew[r];
+                    sh.ren_t[slot] = t;
+                }
             }
         }
         if (err) atomicCAS(&sh.err, 0, err);
@@ -1698,19 +1711,21 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
         PROF_MARK(S, FPH_EVENT, pt);
         // re-prediction of every renewed atom, one wave per atom; a wave takes
         // the next atom when it finishes one (prediction times vary)
-        const int nren = sh.nren;
+        const int nheavy = sh.nren, nlight = sh.nlight;
+        const int nren = nheavy + nlight;             // <= 2 * nex <= 2 * FSEL: the two ends do not meet
+        auto ren_at = [&](int r) { return r < nheavy ? r : 2 * FSEL - nlight + (r - nheavy); };
         err = 0;
         for (;;) {
             int r = 0;
             if (lane == 0) r = atomicAdd(&sh.next_ren, 1);
             r = __shfl(r, 0, 64);
             if (r >= nren || sh.err != 0) break;
-            const int a = sh.ren_a[r];
-            predict_fast<FQ>(F, tab, sh.w[wave], a, sh.ren_t[r], ws, &err);
+            const int q = ren_at(r);
+            predict_fast<FQ>(F, tab, sh.w[wave], sh.ren_a[q], sh.ren_t[q], ws, &err);
             if (__ballot(err != 0)) break;          // uniform across the wave
         }
         for (int r = tid; r < nren; r += 64 * W) {
-            const int b = (sh.ren_a[r] - 1) >> 6;
+            const int b = (sh.ren_a[ren_at(r)] - 1) >> 6;
             if (b >= 0 && b < nblk) atomicOr(&sh.dmask[b >> 5], 1u << (b & 31));
         }
         if (err) atomicCAS(&sh.err, 0, err);

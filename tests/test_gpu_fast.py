@@ -15,7 +15,8 @@ thermostat stream), so the checks are the ones that trajectory allows:
     events executed together commute);
   * batching: systems launched together evolve exactly as each alone;
   * cell arrays: every atom sits in the cell its position indexes;
-  * NVE energy drift of the same order as the reference's own.
+  * NVE energy drift of the same order as the reference's own, and total
+    momentum conserved to rounding.
 """
 import ctypes as C
 
@@ -189,6 +190,27 @@ def test_fast_nve_energy_drift_within_reference(require_gpu):
     # same envelope
     bound = 2 * max(abs(x) for x in ref) + 1.0
     assert max(abs(x) for x in fast) <= bound, (fast, ref)
+
+
+def test_fast_nve_momentum_conserved(require_gpu, mid_nve):
+    """Pair events exchange equal and opposite impulses: over 10^5 events of
+    concurrent execution the total momentum moves by rounding only."""
+    from sdmd_amd.engine import GpuEngine
+
+    def momentum(a):
+        m = a["mass"][1:]
+        return np.array([(m * a[k][1:]).sum() for k in ("vx", "vy", "vz")]), (m * np.abs(a["vx"][1:])).sum()
+
+    eng = GpuEngine([mid_nve], block=512, mode="fast")
+    eng.prepare()
+    p0, scale = momentum(eng.state(0).arr)
+    eng.run_events(100000)
+    st = eng.stats(0)
+    p1, _ = momentum(eng.state(0).arr)
+    eng.close()
+    assert st["error"] == 0 and st["frame"] >= 100000
+    assert st["per_type"][0] > 1000 and st["per_type"][1] > 10000     # collisions and bond events
+    assert np.abs(p1 - p0).max() < 1e-10 * scale, (p0, p1, scale)
 
 
 def _run_to(system, t_end, shape=None, graph=None):
