@@ -21,14 +21,17 @@
 //                bound are re-solved in FP64 with the reference formulas;
 //                the FP64 minimum is the prediction
 //   batches      each round takes every pending event earlier than the
-//                smallest second-earliest time of any 64-atom block (so
-//                the set is exactly a prefix of the event order), sorts it,
-//                and executes concurrently every event whose atoms are at
-//                least 3 cells (4 when a cell crossing is involved) from the
-//                atoms of every earlier event of the set: their cell
-//                neighbourhoods, reads and writes are disjoint, so they
-//                commute with each other and with the events they create.
-//                One wavefront executes one event and re-predicts its atoms.
+//                smallest third-earliest time of any 64-atom block
+//                (second-earliest when that set is too large), so the set
+//                is exactly a prefix of the event order, sorts it, and
+//                executes concurrently every event whose atoms are at least
+//                2 cells (3 when a cell crossing or an H-bond event is
+//                involved) and two cell widths from the atoms of every
+//                earlier event of the set: no event's re-predictions read
+//                the other's atoms, and no event created by one can reach
+//                the other's atoms through a single partner, so they
+//                commute.  One lane executes one event; the re-predictions
+//                are spread over the wavefronts.
 //   tree         the per-block minimum, its atom and the block's second
 //                minimum live in LDS; a round rescans only the blocks whose
 //                atoms changed
@@ -1382,7 +1385,12 @@ struct FShared {
     double red_t[FMAXW], red_t1[FMAXW], red_t3[FMAXW];
     int red_a[FMAXW], red_n[FMAXW];
     double sel_t[FSEL];
-    int sel_a[FSEL], sel_ca[FSEL], sel_cp[FSEL], sel_pcc[FSEL];
+    int sel_a[FSEL], sel_o[FSEL];     // sorted set: atom, index into the pre_* arrays
+    // per selected event before sorting: cells of target and partner (packed
+    // x<<20|y<<10|z), whether it writes beyond its two atoms' state (cell
+    // crossing, H-bond), their positions at the event time
+    int pre_ca[FSEL], pre_cp[FSEL], pre_fl[FSEL];
+    float pre_pa[FSEL][3], pre_pp[FSEL][3];
     unsigned long long conflict;
     int nsort;
     int nsel, nex, ndirty, nren, nlight, next_ren, err, stop, drain;
@@ -1592,17 +1600,27 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
             } else if (lane < nsel) {
                 t = sh.sel_t[lane]; a = sh.sel_a[lane];
             }
-            // cells of each event's target and partner, for the independence
-            // test (loaded before the sort, which carries them along, so the
-            // loads overlap the sorting network)
+            // cells, kind and positions (at the event time) of each event's
+            // target and partner, for the independence test; stored by the
+            // event's lane before the sort, which carries only that lane
+            // index, so the loads overlap the sorting network
             const bool valid = lane < nsel && a >= 1 && a <= S.N;
-            int ca = 0, cp = 0, pcc = 0;
             if (valid) {
-                ca = (G(S.cx)[a] << 20) | (G(S.cy)[a] << 10) | G(S.cz)[a];
-                const int p = G(S.evp)[a];
-                cp = (p > 0 && p <= S.N) ? ((G(S.cx)[p] << 20) | (G(S.cy)[p] << 10) | G(S.cz)[p]) : ca;
-                pcc = (G(S.evtype)[a] == SDMD_PCC);
+                const int p0 = G(S.evp)[a];
+                const int p = (p0 > 0 && p0 <= S.N) ? p0 : a;
+                const int ty = G(S.evtype)[a];
+                sh.pre_ca[lane] = (G(S.cx)[a] << 20) | (G(S.cy)[a] << 10) | G(S.cz)[a];
+                sh.pre_cp[lane] = (G(S.cx)[p] << 20) | (G(S.cy)[p] << 10) | G(S.cz)[p];
+                sh.pre_fl[lane] = (ty == SDMD_PCC || ty == SDMD_HB) ? 1 : 0;
+                const double da = t - G(F.t0)[a], dp = t - G(F.t0)[p];
+                sh.pre_pa[lane][0] = (float)(G(S.x)[a] + G(S.vx)[a] * da);
+                sh.pre_pa[lane][1] = (float)(G(S.y)[a] + G(S.vy)[a] * da);
+                sh.pre_pa[lane][2] = (float)(G(S.z)[a] + G(S.vz)[a] * da);
+                sh.pre_pp[lane][0] = (float)(G(S.x)[p] + G(S.vx)[p] * dp);
+                sh.pre_pp[lane][1] = (float)(G(S.y)[p] + G(S.vy)[p] * dp);
+                sh.pre_pp[lane][2] = (float)(G(S.z)[p] + G(S.vz)[p] * dp);
             }
+            int o = lane;
             // bitonic sort of (t, a) ascending over the wave
 #pragma unroll
             for (int k = 2; k <= 64; k <<= 1) {
@@ -1610,15 +1628,15 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
                 for (int j = k >> 1; j > 0; j >>= 1) {
                     const double to = __shfl_xor(t, j, 64);
                     const int ao = __shfl_xor(a, j, 64);
-                    const int cao = __shfl_xor(ca, j, 64), cpo = __shfl_xor(cp, j, 64), pco = __shfl_xor(pcc, j, 64);
+                    const int oo = __shfl_xor(o, j, 64);
                     const bool up = ((lane & k) == 0);
                     const bool lower = ((lane & j) == 0);
                     const bool less = (to < t) || (to == t && ao < a);
                     const bool take_o = (lower == up) ? less : !less && !(to == t && ao == a);
-                    if (take_o) { t = to; a = ao; ca = cao; cp = cpo; pcc = pco; }
+                    if (take_o) { t = to; a = ao; o = oo; }
                 }
             }
-            if (lane < 64) { sh.sel_a[lane] = a; sh.sel_ca[lane] = ca; sh.sel_cp[lane] = cp; sh.sel_pcc[lane] = pcc; }
+            if (lane < 64) { sh.sel_a[lane] = a; sh.sel_o[lane] = o; }
             if (lane == 0) { sh.nsort = nsel; sh.conflict = 0; }
             if (lane == 0) {
                 F.batches++;
@@ -1631,27 +1649,47 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
         }
         __syncthreads();
         // independence: every pair (l < k) of the sorted set, spread over the
-        // workgroup; event k conflicts when an atom of an earlier event l lies
-        // within D cells (Chebyshev, periodic) of one of its atoms
+        // workgroup; event k conflicts with an earlier event l when an atom
+        // of l lies within D - 1 cells (Chebyshev, periodic) of an atom of k
+        // -- D = 2, so neither event's predictions read the other's atoms;
+        // D = 3 when either writes beyond its atoms' state (a cell crossing
+        // moves an atom into a neighbouring cell, an H-bond event rewrites
+        // its neighbours' bookkeeping) -- or when the two atoms are closer
+        // than two cell widths (>= twice the cut-off, so an event created by
+        // one cannot reach the other's atoms through a single partner)
         {
             const int ns = sh.nsort;
             const int npair = ns * (ns - 1) / 2;
             const int n1 = S.cellnum[1], n2 = S.cellnum[2], n3 = S.cellnum[3];
+            const float bx = (float)S.box[1], by = (float)S.box[2], bz = (float)S.box[3];
+            const float cmin = (float)fmin(S.cellsize[1], fmin(S.cellsize[2], S.cellsize[3]));
+            const float R2 = 4.f * cmin * cmin;
             unsigned long long mine = 0;
             for (int q = tid; q < npair; q += 64 * W) {
                 int k = (int)((1.f + sqrtf(1.f + 8.f * (float)q)) * 0.5f);
                 while (k * (k - 1) / 2 > q) k--;
                 while ((k + 1) * k / 2 <= q) k++;
                 const int l = q - k * (k - 1) / 2;
-                const int D = 3 + ((sh.sel_pcc[k] | sh.sel_pcc[l]) ? 1 : 0);
+                const int ok = sh.sel_o[k], ol = sh.sel_o[l];
+                const int D = 2 + ((sh.pre_fl[ok] | sh.pre_fl[ol]) ? 1 : 0);
                 auto far = [&](int u, int v) {
                     int dx = abs((u >> 20) - (v >> 20)), dy = abs(((u >> 10) & 1023) - ((v >> 10) & 1023)),
                         dz = abs((u & 1023) - (v & 1023));
                     dx = min(dx, n1 - dx); dy = min(dy, n2 - dy); dz = min(dz, n3 - dz);
                     return max(dx, max(dy, dz)) >= D;
                 };
-                const int ka = sh.sel_ca[k], kp = sh.sel_cp[k], la = sh.sel_ca[l], lp = sh.sel_cp[l];
-                if (!(far(ka, la) && far(ka, lp) && far(kp, la) && far(kp, lp))) mine |= 1ull << k;
+                auto apart = [&](const float *u, const float *v) {      // minimum image
+                    float dx = u[0] - v[0], dy = u[1] - v[1], dz = u[2] - v[2];
+                    dx -= bx * rintf(dx / bx);
+                    dy -= by * rintf(dy / by);
+                    dz -= bz * rintf(dz / bz);
+                    return dx * dx + dy * dy + dz * dz >= R2;
+                };
+                const int ka = sh.pre_ca[ok], kp = sh.pre_cp[ok], la = sh.pre_ca[ol], lp = sh.pre_cp[ol];
+                const float *xka = sh.pre_pa[ok], *xkp = sh.pre_pp[ok], *xla = sh.pre_pa[ol], *xlp = sh.pre_pp[ol];
+                const bool indep = far(ka, la) && far(ka, lp) && far(kp, la) && far(kp, lp) &&
+                                   apart(xka, xla) && apart(xka, xlp) && apart(xkp, xla) && apart(xkp, xlp);
+                if (!indep) mine |= 1ull << k;
             }
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) mine |= __shfl_xor(mine, off, 64);
