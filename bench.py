@@ -180,8 +180,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py measures one GPU: the boxes here have a single MI355X; the batch shards "
-              "across GPUs by whole systems, N=%d is not run" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU: the batch shards across GPUs by whole systems (one process "
+              "per GPU), which this benchmark does not launch; N=%d is not run" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()

@@ -1,8 +1,11 @@
-"""The C-ABI library loads and exports every function include/*.h declares
-(no device calls: this runs without a GPU)."""
+"""The C-ABI library loads and exports every function include/*.h declares,
+the headers build as C99, and bench.py refuses more than one GPU (no device
+calls: this runs without a GPU)."""
 import ctypes
 import os
 import re
+import subprocess
+import sys
 
 import pytest
 
@@ -53,7 +56,6 @@ def test_headers_are_plain_c(tmp_path):
     """include/*.h is the boundary a C99 caller (the reference is C99)
     compiles against: the headers must build as C with -std=c99 -pedantic."""
     import shutil
-    import subprocess
 
     gcc = shutil.which("gcc")
     if gcc is None:
@@ -66,3 +68,11 @@ def test_headers_are_plain_c(tmp_path):
     r = subprocess.run([gcc, "-std=c99", "-pedantic", "-Wall", "-Werror", "-fsyntax-only",
                         "-I", os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_bench_runs_on_one_gpu_only():
+    """bench.py --gpus N with N > 1 exits with a message, before touching a device."""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2, r.stderr
+    assert "one GPU" in r.stderr and "N=2" in r.stderr
