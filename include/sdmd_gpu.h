@@ -157,8 +157,8 @@ int sdmd_gpu_energy(sdmd_gpu_ctx *ctx, double *kinetic, double *potential);
 int sdmd_gpu_set_temperature(sdmd_gpu_ctx *ctx, int32_t sys, double temperature);
 int sdmd_gpu_fast_stats_get(sdmd_gpu_ctx *ctx, int32_t sys, sdmd_gpu_fast_stats *out);
 /* Fast mode, batching form: time every workgroup shape (4, 8, 12 or 16
- * wavefronts per system, i.e. concurrent events per round and pass, and 2 or
- * 4 candidates staged per lane) for trial_ms of device time on the live
+ * wavefronts per system, i.e. re-predictions in flight at once, and 2 or 4
+ * candidates staged per lane) for trial_ms of device time on the live
  * systems and keep the fastest; the systems advance meanwhile.  No reference
  * counterpart (the reference is single-threaded). */
 int sdmd_gpu_autotune(sdmd_gpu_ctx *ctx, double trial_ms, int32_t *block_out, int32_t *depth_out);

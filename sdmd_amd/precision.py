@@ -15,7 +15,7 @@ in NVE (no thermostat), so total energy is the conserved quantity.
   engine, which is bit-exact with the reference, over the same number of
   executed events.  The reference's NVE energy moves in steps when H-bond
   bookkeeping (HBEnergyChange, Events.c:1027-1095) sums shell energies in
-  cell-list order; both engines are judged against that envelope.
+  cell-list order; both engines are measured against that envelope.
 
 Prints one JSON object.
 """
