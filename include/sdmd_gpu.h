@@ -61,8 +61,8 @@ enum {
 typedef struct {
     int32_t mode;          /* SDMD_MODE_PLAIN or SDMD_MODE_FAST */
     int32_t block;         /* threads per system workgroup: plain 64..1024 (power of two),
-                              fast 64, 256, 512 or 1024 (one event per wavefront; the
-                              autotuner may also pick 768); 0 = 256 */
+                              fast 64, 256, 512 or 1024 (one re-prediction per wavefront
+                              at a time; the autotuner may also pick 768); 0 = 256 */
     int32_t trace;         /* plain: record every pop (parity checks) */
     int32_t profile;       /* plain, bit 0: accumulate per-phase device time */
     int64_t trace_capacity;

@@ -1712,7 +1712,7 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
         }
         __syncthreads();
         PROF_MARK(S, FPH_SELECT, pt);
-        // ---- execution: one wave per event
+        // ---- execution: one lane per event
         const int nex = sh.nex;
         if (nex < 1) {                                     // no selectable event: stop with an error
             if (tid == 0) sh.err = FERR_SELECT;
