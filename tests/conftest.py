@@ -54,3 +54,38 @@ def gpu_available():
 def require_gpu():
     if not gpu_available():
         pytest.fail("no HIP device visible: GPU tests must run on an MI355X")
+
+
+STATE_FIELDS = ("x", "y", "z", "vx", "vy", "vz", "ev_partner", "ev_type", "ev_sub", "ev_time", "cellx", "celly",
+                "cellz")
+
+
+def fast_schedule_states(system, events, block=512):
+    """The fast engine run to one time twice: one event per round, and with
+    its full concurrency.  Returns (state, stats, fast_stats) of each run;
+    concurrent events commute, so the states must be bit-identical."""
+    from sdmd_amd.engine import GpuEngine
+
+    eng = GpuEngine([system], block=block, mode="fast")
+    eng.prepare()
+    eng.run_events(events)
+    t_end = 0.9 * eng.fast_stats(0)["clock"]
+    eng.close()
+    out = []
+    for mb in (1, 0):
+        eng = GpuEngine([system], block=block, mode="fast", max_batch=mb)
+        eng.prepare()
+        eng.run_until(t_end)
+        out.append((eng.state(0), eng.stats(0), eng.fast_stats(0)))
+        eng.close()
+    return out
+
+
+def assert_same_schedule_result(runs, min_frames, min_per_round=2):
+    (a, sa, fa), (b, sb, fb) = runs
+    assert sa["error"] == 0 and sb["error"] == 0
+    assert fa["causal"] == 0 and fb["causal"] == 0
+    assert sa["frame"] == sb["frame"] and sa["frame"] > min_frames, (sa["frame"], sb["frame"])
+    assert fb["batches"] * min_per_round < sb["pops"]     # events per round ran concurrently
+    for f in STATE_FIELDS:
+        assert np.array_equal(a.arr[f], b.arr[f]), f

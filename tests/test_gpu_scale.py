@@ -169,3 +169,11 @@ def test_large_system_fast_screen_agrees_with_fp64(require_gpu, large):
     assert fs["validated"] > 100000
     assert fs["agree"] == fs["validated"], fs
     eng.close()
+
+
+def test_large_system_fast_schedule_invariance(require_gpu, large):
+    """At full size (about 24 concurrent events per round) one event per
+    round and full concurrency reach bit-identical states."""
+    from conftest import assert_same_schedule_result, fast_schedule_states
+
+    assert_same_schedule_result(fast_schedule_states(large, 40000), 20000, min_per_round=10)

@@ -172,3 +172,12 @@ def test_remd_loop_exchanges_temperatures(require_gpu):
     assert r.counts[0::2].sum() == 6 * n
     assert r.counts[1::2].sum() > 0        # neighbouring temperatures this close do swap
     r.close()
+
+
+def test_fast_wall_schedule_invariance(require_gpu):
+    """In the cylinder (wall events, periodic along x only) one event per
+    round and full concurrency reach bit-identical states."""
+    from conftest import assert_same_schedule_result, fast_schedule_states
+
+    s, _ = golden_system("cyl8x20")
+    assert_same_schedule_result(fast_schedule_states(s, 30000), 15000)
