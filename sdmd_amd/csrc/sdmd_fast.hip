@@ -23,8 +23,8 @@
 //   batches      each round takes every pending event earlier than the
 //                smallest third-earliest time of any 64-atom block
 //                (second-earliest when that set is too large), so the set
-//                is exactly a prefix of the event order, sorts it, and
-//                executes concurrently every event whose atoms are at least
+//                is exactly a prefix of the event order, and executes
+//                concurrently every event whose atoms are at least
 //                2 cells (3 when a cell crossing or an H-bond event is
 //                involved) and two cell widths from the atoms of every
 //                earlier event of the set: no event's re-predictions read
@@ -1385,10 +1385,10 @@ struct FShared {
     double red_t[FMAXW], red_t1[FMAXW], red_t3[FMAXW];
     int red_a[FMAXW], red_n[FMAXW];
     double sel_t[FSEL];
-    int sel_a[FSEL], sel_o[FSEL];     // sorted set: atom, index into the pre_* arrays
-    // per selected event before sorting: cells of target and partner (packed
-    // x<<20|y<<10|z), whether it writes beyond its two atoms' state (cell
-    // crossing, H-bond), their positions at the event time
+    int sel_a[FSEL], sel_rank[FSEL];  // selected set: atom, number of earlier events in the set
+    // per selected event: cells of target and partner (packed x<<20|y<<10|z),
+    // whether it writes beyond its two atoms' state (cell crossing, H-bond),
+    // their positions at the event time
     int pre_ca[FSEL], pre_cp[FSEL], pre_fl[FSEL];
     float pre_pa[FSEL][3], pre_pp[FSEL][3];
     unsigned long long conflict;
@@ -1601,9 +1601,7 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
                 t = sh.sel_t[lane]; a = sh.sel_a[lane];
             }
             // cells, kind and positions (at the event time) of each event's
-            // target and partner, for the independence test; stored by the
-            // event's lane before the sort, which carries only that lane
-            // index, so the loads overlap the sorting network
+            // target and partner, for the independence test
             const bool valid = lane < nsel && a >= 1 && a <= S.N;
             if (valid) {
                 const int p0 = G(S.evp)[a];
@@ -1620,23 +1618,9 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
                 sh.pre_pp[lane][1] = (float)(G(S.y)[p] + G(S.vy)[p] * dp);
                 sh.pre_pp[lane][2] = (float)(G(S.z)[p] + G(S.vz)[p] * dp);
             }
-            int o = lane;
-            // bitonic sort of (t, a) ascending over the wave
-#pragma unroll
-            for (int k = 2; k <= 64; k <<= 1) {
-#pragma unroll
-                for (int j = k >> 1; j > 0; j >>= 1) {
-                    const double to = __shfl_xor(t, j, 64);
-                    const int ao = __shfl_xor(a, j, 64);
-                    const int oo = __shfl_xor(o, j, 64);
-                    const bool up = ((lane & k) == 0);
-                    const bool lower = ((lane & j) == 0);
-                    const bool less = (to < t) || (to == t && ao < a);
-                    const bool take_o = (lower == up) ? less : !less && !(to == t && ao == a);
-                    if (take_o) { t = to; a = ao; o = oo; }
-                }
-            }
-            if (lane < 64) { sh.sel_a[lane] = a; sh.sel_o[lane] = o; }
+            sh.sel_a[lane] = a;
+            sh.sel_t[lane] = t;
+            sh.sel_rank[lane] = 0;
             if (lane == 0) { sh.nsort = nsel; sh.conflict = 0; }
             if (lane == 0) {
                 F.batches++;
@@ -1648,9 +1632,10 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
             }
         }
         __syncthreads();
-        // independence: every pair (l < k) of the sorted set, spread over the
-        // workgroup; event k conflicts with an earlier event l when an atom
-        // of l lies within D - 1 cells (Chebyshev, periodic) of an atom of k
+        // independence: every pair of the set, spread over the workgroup (no
+        // sorting: the pair's time order is compared directly); the later
+        // event k of a pair conflicts with the earlier one l when an atom of
+        // l lies within D - 1 cells (Chebyshev, periodic) of an atom of k
         // -- D = 2, so neither event's predictions read the other's atoms;
         // D = 3 when either writes beyond its atoms' state (a cell crossing
         // moves an atom into a neighbouring cell, an H-bond event rewrites
@@ -1666,12 +1651,14 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
             const float R2 = 4.f * cmin * cmin;
             unsigned long long mine = 0;
             for (int q = tid; q < npair; q += 64 * W) {
-                int k = (int)((1.f + sqrtf(1.f + 8.f * (float)q)) * 0.5f);
-                while (k * (k - 1) / 2 > q) k--;
-                while ((k + 1) * k / 2 <= q) k++;
-                const int l = q - k * (k - 1) / 2;
-                const int ok = sh.sel_o[k], ol = sh.sel_o[l];
-                const int D = 2 + ((sh.pre_fl[ok] | sh.pre_fl[ol]) ? 1 : 0);
+                int i = (int)((1.f + sqrtf(1.f + 8.f * (float)q)) * 0.5f);
+                while (i * (i - 1) / 2 > q) i--;
+                while ((i + 1) * i / 2 <= q) i++;
+                const int j = q - i * (i - 1) / 2;
+                const bool jfirst = sh.sel_t[j] < sh.sel_t[i] || (sh.sel_t[j] == sh.sel_t[i] && sh.sel_a[j] < sh.sel_a[i]);
+                const int k = jfirst ? i : j, l = jfirst ? j : i;      // l is the earlier event
+                atomicAdd(&sh.sel_rank[k], 1);
+                const int D = 2 + ((sh.pre_fl[k] | sh.pre_fl[l]) ? 1 : 0);
                 auto far = [&](int u, int v) {
                     int dx = abs((u >> 20) - (v >> 20)), dy = abs(((u >> 10) & 1023) - ((v >> 10) & 1023)),
                         dz = abs((u & 1023) - (v & 1023));
@@ -1685,8 +1672,8 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
                     dz -= bz * rintf(dz / bz);
                     return dx * dx + dy * dy + dz * dz >= R2;
                 };
-                const int ka = sh.pre_ca[ok], kp = sh.pre_cp[ok], la = sh.pre_ca[ol], lp = sh.pre_cp[ol];
-                const float *xka = sh.pre_pa[ok], *xkp = sh.pre_pp[ok], *xla = sh.pre_pa[ol], *xlp = sh.pre_pp[ol];
+                const int ka = sh.pre_ca[k], kp = sh.pre_cp[k], la = sh.pre_ca[l], lp = sh.pre_cp[l];
+                const float *xka = sh.pre_pa[k], *xkp = sh.pre_pp[k], *xla = sh.pre_pa[l], *xlp = sh.pre_pp[l];
                 const bool indep = far(ka, la) && far(ka, lp) && far(kp, la) && far(kp, lp) &&
                                    apart(xka, xla) && apart(xka, xlp) && apart(xkp, xla) && apart(xkp, xlp);
                 if (!indep) mine |= 1ull << k;
@@ -1699,8 +1686,8 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
         if (wave == 0) {
             const int ns = sh.nsort;
             const int a = sh.sel_a[lane];
-            const bool ok = lane < ns && a >= 1 && a <= S.N && !((sh.conflict >> lane) & 1);
-            // keep sorted order; cap the batch
+            // the independent events among the `cap` earliest of the set
+            const bool ok = lane < ns && a >= 1 && a <= S.N && !((sh.conflict >> lane) & 1) && sh.sel_rank[lane] < cap;
             const unsigned long long okm = __ballot(ok);
             const int rank = __popcll(okm & ((1ULL << lane) - 1));
             if (ok && rank < cap) sh.ex_a[rank] = a;
