@@ -1392,7 +1392,7 @@ struct FShared {
     int pre_ca[FSEL], pre_cp[FSEL], pre_fl[FSEL];
     float pre_pa[FSEL][3], pre_pp[FSEL][3];
     unsigned long long conflict;
-    int nsort;
+    int nset;                          // events in the selected set
     int nsel, nex, ndirty, nren, nlight, next_ren, err, stop, drain;
     long long drain_end;
     int ex_a[FSEL];
@@ -1621,7 +1621,7 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
             sh.sel_a[lane] = a;
             sh.sel_t[lane] = t;
             sh.sel_rank[lane] = 0;
-            if (lane == 0) { sh.nsort = nsel; sh.conflict = 0; }
+            if (lane == 0) { sh.nset = nsel; sh.conflict = 0; }
             if (lane == 0) {
                 F.batches++;
                 if (single) F.single++;
@@ -1643,7 +1643,7 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
         // than two cell widths (>= twice the cut-off, so an event created by
         // one cannot reach the other's atoms through a single partner)
         {
-            const int ns = sh.nsort;
+            const int ns = sh.nset;
             const int npair = ns * (ns - 1) / 2;
             const int n1 = S.cellnum[1], n2 = S.cellnum[2], n3 = S.cellnum[3];
             const float bx = (float)S.box[1], by = (float)S.box[2], bz = (float)S.box[3];
@@ -1684,7 +1684,7 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
         }
         __syncthreads();
         if (wave == 0) {
-            const int ns = sh.nsort;
+            const int ns = sh.nset;
             const int a = sh.sel_a[lane];
             // the independent events among the `cap` earliest of the set
             const bool ok = lane < ns && a >= 1 && a <= S.N && !((sh.conflict >> lane) & 1) && sh.sel_rank[lane] < cap;
