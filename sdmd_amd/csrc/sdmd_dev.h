@@ -215,11 +215,21 @@ static __device__ __forceinline__ int shell_lookup(const Sys &S, const sdmd_pair
         // Every step list is strictly increasing and non-zero (checked at
         // upload), so the walk below stops at the first k with
         // d2 < d_k + tol, i.e. at the number of radii d2 has passed.  The
-        // radii are read with independent loads.
+        // radii are read four at a time with unconditional loads (indices
+        // clamped to the pair's last step), so each group costs one memory
+        // round trip rather than one per radius.
         int ks = 0;
 #pragma unroll
-        for (int k = 0; k < 16; k++)
-            if (k < nstep) ks += (d2 >= G(S.sd)[step0 + k] + tol) ? 1 : 0;
+        for (int k0 = 0; k0 < 16; k0 += 4) {
+            if (k0 < nstep) {
+                double d[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) d[u] = G(S.sd)[step0 + (k0 + u < nstep ? k0 + u : nstep - 1)];
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (k0 + u < nstep) ks += (d2 >= d[u] + tol) ? 1 : 0;
+            }
+        }
         if (ks > 0) {
             f.lo = G(S.sd)[step0 + ks - 1];
             f.lopot = G(S.se)[step0 + ks - 1];

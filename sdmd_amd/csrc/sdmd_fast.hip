@@ -368,17 +368,27 @@ __device__ __forceinline__ Scr screen_shell(const Shell &c, const R &rad, bool c
     o.t = FINF; o.e = 0; o.flag = false;
     const float band = 2e-5f * rr + 1e-5f;
     if (rr < c.dmin + band) { o.flag = true; return o; }   // core boundary or inside the core
-    // count the radii passed (no early exit: the loads issue together)
+    // count the radii passed (no early exit; four radii read at a time with
+    // unconditional loads clamped to the last step, so a group costs one
+    // LDS round trip)
     int ks = 0;
     bool near = false;
     float lo = c.dmin, hi = FINF;
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        if (k < c.nstep) {
-            const float d = rad(k);
-            near |= fabsf(rr - d) < band;
-            if (rr > d) { ks++; lo = d; }
-            else if (hi == FINF) hi = d;
+    for (int k0 = 0; k0 < 16; k0 += 4) {
+        if (k0 < c.nstep) {
+            float dq[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) dq[u] = rad(k0 + u < c.nstep ? k0 + u : c.nstep - 1);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (k0 + u < c.nstep) {
+                    const float d = dq[u];
+                    near |= fabsf(rr - d) < band;
+                    if (rr > d) { ks++; lo = d; }
+                    else if (hi == FINF) hi = d;
+                }
+            }
         }
     }
     if (near) { o.flag = true; return o; }
