@@ -54,6 +54,9 @@ struct oracle {
     int error;
     long rand_calls;
     uint32_t seed;
+    /* per atom: 1 when its current event record was copied from its
+       partner's by CheckDuplication rather than solved (checker output) */
+    signed char *inh;
 };
 
 /* the neighbour-cell visiting order, DMD.c:118-120 */
@@ -68,6 +71,7 @@ oracle *oracle_open(const sdmd_system_desc *d) {
     oracle *o = calloc(1, sizeof *o);
     int N = d->natom, N1 = N + 1;
     o->N = N;
+    o->inh = calloc((size_t)N1, 1);
     for (int k = 0; k < 3; k++) {
         o->cellnum[k + 1] = d->cellnum[k];
         o->cellsize[k + 1] = d->cellsize[k];
@@ -125,6 +129,7 @@ oracle *oracle_open(const sdmd_system_desc *d) {
 
 void oracle_close(oracle *o) {
     if (!o) return;
+    free(o->inh);
     void *p[] = {o->x, o->y, o->z, o->vx, o->vy, o->vz, o->mass, o->type, o->aatype, o->protein,
                  o->aanum, o->cx, o->cy, o->cz, o->role, o->hbnb, o->hbbond, o->hbitype, o->nbp,
                  o->nbnum, o->nbstat, o->dynj, o->dynb, o->evp, o->evtype, o->evsub, o->evc,
@@ -377,6 +382,7 @@ static void init_event(ev_t *e) { e->time = INFINIT; e->pot = 0; e->type = SDMD_
 
 static int job_assign(oracle *o, int i, int j, const ev_t *e, int type) {   /* JobAssign 1140-1158 */
     if (e->time > 0 && o->evt[i] > e->time) {
+        o->inh[i] = 0;
         o->evpot[i] = e->pot;
         o->evsub[i] = e->sub;
         o->evt[i] = e->time;
@@ -390,6 +396,7 @@ static int job_assign(oracle *o, int i, int j, const ev_t *e, int type) {   /* J
 
 static int check_dup(oracle *o, int i, int j, int type) {   /* CheckDuplication 1101-1117 */
     if (o->evp[j] == i && o->evtype[j] == type && o->evpc[j] == o->evc[i] && o->evt[i] > o->evt[j]) {
+        o->inh[i] = 1;
         o->evp[i] = j;
         o->evpot[i] = o->evpot[j];
         o->evsub[i] = o->evsub[j];
@@ -718,6 +725,7 @@ int oracle_predict(oracle *o, const int32_t *renew) {
         o->evtype[i] = SDMD_INVD;
         o->evt[i] = INFINIT;
         o->evp[i] = INVALID;
+        o->inh[i] = 0;
     }
     for (int n = 1; renew[n]; n++) {
         int i = renew[n];
@@ -767,6 +775,10 @@ int oracle_create_cbt(oracle *o) {
     o->count = 0;
     for (int i = 1; i <= o->N; i++) cbt_insert_run(o, i);
     return 0;
+}
+
+void oracle_inherited(const oracle *o, int8_t *out) {
+    memcpy(out, o->inh, (size_t)o->N + 1);
 }
 
 int oracle_bulk(oracle *o) {

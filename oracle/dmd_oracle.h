@@ -55,6 +55,10 @@ int64_t oracle_rand_calls(const oracle *o);
 int oracle_predict(oracle *o, const int32_t *renew);
 /* Predict over 1..N then CreateCBT (SGThread.c:47-62) */
 int oracle_bulk(oracle *o);
+/* out[i] (natom+1 entries) = 1 when atom i's current event record was copied
+   from its partner's by CheckDuplication (TimePrediction.c:1101-1117) rather
+   than solved for atom i itself */
+void oracle_inherited(const oracle *o, int8_t *out);
 int oracle_create_cbt(oracle *o);
 /* WallTime for atom i alone, leaving the state unchanged: 1 and the event
  * (time from now, potential, subtype) when a wall event is offered, else 0 */

@@ -681,22 +681,30 @@ static __device__ void hb_event(Sys &S, int i, int j, int nbi, int nbj, int *err
 // Sum of two per-thread values over a workgroup of B threads, in a fixed
 // order (each wave by butterfly, then the waves in index order); thread 0
 // writes the two sums to out[0], out[1].
+// Workgroup sums of a and b and the OR of a per-thread error flag, written
+// to out[blockIdx.x], out[gridDim.x + blockIdx.x] and
+// out[2 * gridDim.x + blockIdx.x] (three planes: kinetic, potential, error).
 template <int B>
-static __device__ void block_sum2(double a, double b, double *out) {
+static __device__ void block_sum3(double a, double b, int err, double *out) {
     __shared__ double red[2 * (B / 64)];
+    __shared__ int any_err;
+    if (threadIdx.x == 0) any_err = 0;
+    __syncthreads();
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         a += __shfl_xor(a, off, 64);
         b += __shfl_xor(b, off, 64);
     }
+    if (err) atomicOr(&any_err, 1);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (lane == 0) { red[2 * wave] = a; red[2 * wave + 1] = b; }
     __syncthreads();
     if (threadIdx.x == 0) {
         double sa = 0, sb = 0;
         for (int w = 0; w < B / 64; w++) { sa += red[2 * w]; sb += red[2 * w + 1]; }
-        G(out)[0] = sa;
-        G(out)[1] = sb;
+        G(out)[blockIdx.x] = sa;
+        G(out)[gridDim.x + blockIdx.x] = sb;
+        G(out)[2 * gridDim.x + blockIdx.x] = any_err ? 1.0 : 0.0;
     }
 }
 
@@ -718,7 +726,7 @@ struct sdmd_gpu_ctx {
     void **allocs;
     int nalloc, capalloc;
     int *dev_renew;
-    double *dev_energy;        // 2 doubles per system (kinetic, potential)
+    double *dev_energy;        // 3 planes of nsys doubles (kinetic, potential, error flag)
     double last_ms;
 };
 
@@ -753,6 +761,6 @@ int sdmd_fast_stats(sdmd_gpu_ctx *c, int s, sdmd_gpu_fast_stats *out);
 int sdmd_fast_sync(sdmd_gpu_ctx *c);
 int sdmd_fast_run_graph(sdmd_gpu_ctx *c, int launches, double max_ms);
 int sdmd_fast_autotune(sdmd_gpu_ctx *c, double trial_ms, int32_t *block_out, int32_t *depth_out);
-int sdmd_fast_energy(sdmd_gpu_ctx *c, double *kinetic, double *potential);
+int sdmd_fast_energy_launch(sdmd_gpu_ctx *c, double *planes);
 int sdmd_fast_set_temperature(sdmd_gpu_ctx *c, int s, double T);
 void sdmd_fast_close(sdmd_gpu_ctx *c);

@@ -56,7 +56,12 @@ constexpr int FMAXW = 16;         // wavefronts per workgroup
 constexpr long long FDRAIN = 1 << 20;   // bound on the pops of a closing drain
 constexpr float FINF = 3.0e38f;
 
-enum { FERR_SELECT = 20, FERR_SLOT = 21, FERR_ROW = 22, FERR_CAND = 23, FERR_NBLK = 24 };
+// FERR_CAUSAL: an atom was read at a local time later than the reader's
+// (the independence rule failed); FERR_DRAIN: a closing drain reached its
+// pop bound with events still pending up to the stop time, so the state is
+// not a snapshot at one time
+enum { FERR_SELECT = 20, FERR_SLOT = 21, FERR_ROW = 22, FERR_CAND = 23, FERR_NBLK = 24, FERR_CAUSAL = 25,
+       FERR_DRAIN = 26 };
 
 // phases timed (by wavefront 0) when profiling is on
 enum { FPH_SELECT, FPH_EXEC, FPH_STAGE, FPH_SCREEN, FPH_CONFIRM, FPH_OTHER, FPH_TREE, FPH_EVENT, FPH_BOND,
@@ -296,9 +301,10 @@ __device__ __forceinline__ int fin_nb_list(const FAtom &A, int j) {
 }
 
 // pair kinematics for atom A (at its time) and j (local state advanced to A.t)
-__device__ __forceinline__ void kin_local(const FSys &F, const FAtom &A, int j, Kin &k) {
+__device__ __forceinline__ void kin_local(const FSys &F, const FAtom &A, int j, Kin &k, bool *late = nullptr) {
     const Sys &S = F.s;
     const double dt = A.t - G(F.t0)[j];
+    if (late) *late = dt < 0;                  // j's state is later than A's time
     const double vxj = G(S.vx)[j], vyj = G(S.vy)[j], vzj = G(S.vz)[j];
     calc_params_at(S, A.x, A.y, A.z, A.vx, A.vy, A.vz, A.cx, A.cy, A.cz, j, G(S.x)[j] + vxj * dt, G(S.y)[j] + vyj * dt,
                    G(S.z)[j] + vzj * dt, vxj, vyj, vzj, k);
@@ -310,13 +316,13 @@ __device__ __forceinline__ void kin_local(const FSys &F, const FAtom &A, int j, 
 // ToolFunctions.c:278-292), known from the screen, so the table entry loads
 // with j's state instead of after it
 __device__ double coli64(const FSys &F, const FAtom &A, int j, int bits, int rp, double calib, double &pot, int &sub,
-                         bool &reached, int *err) {
+                         bool &reached, bool &late, int *err) {
     const Sys &S = F.s;
     reached = false;
     pot = 0;
     sub = SDMD_INVALID_INTER;
     Kin k;
-    kin_local(F, A, j, k);
+    kin_local(F, A, j, k, &late);
     if (!(k.rr <= calib)) return __builtin_inf();
     FP f;
     int cs;
@@ -861,8 +867,9 @@ __device__ __forceinline__ void collision_fast(FSys &F, const ScrTab &T, FWave &
     auto confirm = [&](int k, const CandS &o, bool need) {
         double pot;
         int sub;
-        bool reached;
-        const double t = coli64(F, A, o.j, o.bits, o.rp, calib64, pot, sub, reached, err);
+        bool reached, late;
+        const double t = coli64(F, A, o.j, o.bits, o.rp, calib64, pot, sub, reached, late, err);
+        my_causal += late ? 1 : 0;                 // FP64 check of the screen's FP32 time order
         if (need) {
             my_conf++;
             if (t < mine.t) { mine.t = t; mine.pot = pot; mine.sub = sub; mine.partner = o.j; mink = k; myt32 = o.t; }
@@ -945,7 +952,10 @@ __device__ __forceinline__ void collision_fast(FSys &F, const ScrTab &T, FWave &
     nscreen += wave_sum_i(my_screen);
     nconf += wave_sum_i(my_conf);
     my_causal = wave_sum_i(my_causal);
-    if (my_causal && lane == 0) cnt_add(F.causal, my_causal);
+    if (my_causal && lane == 0) {
+        cnt_add(F.causal, my_causal);
+        *err = FERR_CAUSAL;
+    }
     // wave argmin (first minimal candidate)
     double tw = mine.t;
     int kw = mink;
@@ -1302,7 +1312,10 @@ __device__ __forceinline__ int exec_fast(FSys &F, int a, double t, int *renew, i
         return 1;
     }
     if (partner) {
-        if (G(F.t0)[partner] > t) cnt_add(F.causal, 1);
+        if (G(F.t0)[partner] > t) {
+            cnt_add(F.causal, 1);
+            *err = FERR_CAUSAL;
+        }
         advance(F, partner, t);
     }
     const int ty = G(S.evtype)[a];
@@ -1566,8 +1579,9 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
                 if (F.tlast < F.t_stop) F.t_stop = F.tlast;
                 sh.drain_end = S.pops + FDRAIN;
             }
-            if (sh.err || !(tmin < INFINIT) || !(tmin <= F.t_stop) || amin < 1 || amin > S.N ||
-                (sh.drain && S.pops >= sh.drain_end))
+            const bool pending = tmin < INFINIT && tmin <= F.t_stop;
+            if (sh.drain && S.pops >= sh.drain_end && pending && !sh.err) sh.err = FERR_DRAIN;
+            if (sh.err || !pending || amin < 1 || amin > S.N || (sh.drain && S.pops >= sh.drain_end))
                 sh.stop = 1;
             sh.nsel = 0;
         }
@@ -2059,19 +2073,25 @@ static int budget_ticks(double max_ms, long long *out) {
     return SDMD_OK;
 }
 
+// every system's device error word (0 = none) after a sync
+static int fast_errors(sdmd_gpu_ctx *c) {
+    for (int s = 0; s < c->nsys; s++)
+        if (c->fast->host[s].s.error) return SDMD_ESTATE;
+    return SDMD_OK;
+}
+
 int sdmd_fast_prepare(sdmd_gpu_ctx *c) {
     FastState *fs = c->fast;
     HIPV(hipEventRecord(c->ev0, c->stream));
     FDISPATCH(k_fast_prepare, fs->dev);
+    if (hipGetLastError() != hipSuccess) return SDMD_EDEVICE;   // launch rejected (shape, resources)
     HIPV(hipEventRecord(c->ev1, c->stream));
     if (hipStreamSynchronize(c->stream) != hipSuccess) return SDMD_EDEVICE;
     float ms = 0;
     HIPV(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
     if (fast_sync_from(c)) return SDMD_EDEVICE;
-    for (int s = 0; s < c->nsys; s++)
-        if (fs->host[s].s.error) return SDMD_ESTATE;
-    return SDMD_OK;
+    return fast_errors(c);
 }
 
 int sdmd_fast_run(sdmd_gpu_ctx *c, long long pops, double max_ms, double t_end) {
@@ -2080,15 +2100,14 @@ int sdmd_fast_run(sdmd_gpu_ctx *c, long long pops, double max_ms, double t_end) 
     if (budget_ticks(max_ms, &budget)) return SDMD_EDEVICE;
     HIPV(hipEventRecord(c->ev0, c->stream));
     launch_run(c, pops, budget, t_end);
+    if (hipGetLastError() != hipSuccess) return SDMD_EDEVICE;
     HIPV(hipEventRecord(c->ev1, c->stream));
     if (hipStreamSynchronize(c->stream) != hipSuccess) return SDMD_EDEVICE;
     float ms = 0;
     HIPV(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
     if (fast_sync_from(c)) return SDMD_EDEVICE;
-    for (int s = 0; s < c->nsys; s++)
-        if (fs->host[s].s.error) return SDMD_ESTATE;
-    return SDMD_OK;
+    return fast_errors(c);
 }
 
 // positions at the clock (earliest pending event), event times relative to it
@@ -2097,6 +2116,7 @@ int sdmd_fast_download(sdmd_gpu_ctx *c, int s, sdmd_system_desc *out) {
     if (fast_sync_from(c)) return SDMD_EDEVICE;
     const FSys &F = fs->host[s];
     const Sys &S = F.s;
+    if (S.error) return SDMD_ESTATE;         // e.g. an incomplete drain: not a snapshot at one time
     const int N = S.N;
     const size_t N1 = (size_t)N + 1;
     double *t0 = (double *)malloc(sizeof(double) * N1);
@@ -2202,22 +2222,15 @@ __global__ __launch_bounds__(64 * W) void k_fast_energy(FSys *systems, double *o
             }
         }
     }
-    block_sum2<64 * W>(ke, pe, out + 2 * blockIdx.x);
+    block_sum3<64 * W>(ke, pe, err || S.error, out);
 }
 
-int sdmd_fast_energy(sdmd_gpu_ctx *c, double *kinetic, double *potential) {
+// energies of every system into out's three planes (kinetic, potential,
+// error flag; energy_planes_to_host reads them back)
+int sdmd_fast_energy_launch(sdmd_gpu_ctx *c, double *out) {
     FastState *fs = c->fast;
-    FDISPATCH(k_fast_energy, fs->dev, c->dev_energy);
-    double *h = (double *)malloc(sizeof(double) * 2 * (size_t)c->nsys);
-    if (!h) return SDMD_ENOMEM;
-    int rc = hipGetLastError() == hipSuccess ? SDMD_OK : SDMD_EDEVICE;
-    if (!rc && (hipMemcpyAsync(h, c->dev_energy, sizeof(double) * 2 * (size_t)c->nsys, hipMemcpyDeviceToHost,
-                               c->stream) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess))
-        rc = SDMD_EDEVICE;
-    if (!rc)
-        for (int s = 0; s < c->nsys; s++) { kinetic[s] = h[2 * s]; potential[s] = h[2 * s + 1]; }
-    free(h);
-    return rc;
+    FDISPATCH(k_fast_energy, fs->dev, out);
+    return hipGetLastError() == hipSuccess ? SDMD_OK : SDMD_EDEVICE;
 }
 
 // the thermostat's target temperature of one system (REMD exchanges set it)
@@ -2261,9 +2274,20 @@ int sdmd_fast_run_graph(sdmd_gpu_ctx *c, int launches, double max_ms) {
         if (fs->gexec) { HIPV(hipGraphExecDestroy(fs->gexec)); fs->gexec = nullptr; }
         if (fs->graph) { HIPV(hipGraphDestroy(fs->graph)); fs->graph = nullptr; }
         if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return SDMD_EDEVICE;
-        for (int k = 0; k < launches; k++) launch_run(c, 1LL << 60, budget, INFINIT);
-        if (hipStreamEndCapture(c->stream, &fs->graph) != hipSuccess) return SDMD_EDEVICE;
-        if (hipGraphInstantiate(&fs->gexec, fs->graph, nullptr, nullptr, 0) != hipSuccess) return SDMD_EDEVICE;
+        bool launched = true;
+        for (int k = 0; k < launches; k++) {
+            launch_run(c, 1LL << 60, budget, INFINIT);
+            launched = launched && hipGetLastError() == hipSuccess;
+        }
+        const bool captured = hipStreamEndCapture(c->stream, &fs->graph) == hipSuccess;
+        if (!launched || !captured) {
+            if (fs->graph) { HIPV(hipGraphDestroy(fs->graph)); fs->graph = nullptr; }
+            return SDMD_EDEVICE;
+        }
+        if (hipGraphInstantiate(&fs->gexec, fs->graph, nullptr, nullptr, 0) != hipSuccess) {
+            fs->gexec = nullptr;
+            return SDMD_EDEVICE;
+        }
         fs->g_launches = key;
         fs->g_budget = budget;
     }
@@ -2275,9 +2299,7 @@ int sdmd_fast_run_graph(sdmd_gpu_ctx *c, int launches, double max_ms) {
     HIPV(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
     if (fast_sync_from(c)) return SDMD_EDEVICE;
-    for (int s = 0; s < c->nsys; s++)
-        if (fs->host[s].s.error) return SDMD_ESTATE;
-    return SDMD_OK;
+    return fast_errors(c);
 }
 
 // Time each workgroup shape (wavefronts per system x candidates staged per

@@ -887,7 +887,7 @@ __global__ __launch_bounds__(B) void k_energy(Sys *systems, double *out) {
             }
         }
     }
-    block_sum2<B>(ke, pe, out + 2 * blockIdx.x);
+    block_sum3<B>(ke, pe, err || S.error, out);
 }
 
 template <int B>
@@ -1099,8 +1099,8 @@ static void launch_run(sdmd_gpu_ctx *c, long long n, long long budget, double t_
     hipLaunchKernelGGL((k_run<B>), dim3(c->nsys), dim3(B), 0, c->stream, c->dev_sys, n, budget, t_end);
 }
 template <int B>
-static void launch_energy(sdmd_gpu_ctx *c) {
-    hipLaunchKernelGGL((k_energy<B>), dim3(c->nsys), dim3(B), 0, c->stream, c->dev_sys, c->dev_energy);
+static void launch_energy(sdmd_gpu_ctx *c, double *planes) {
+    hipLaunchKernelGGL((k_energy<B>), dim3(c->nsys), dim3(B), 0, c->stream, c->dev_sys, planes);
 }
 template <int B>
 static void launch_op(sdmd_gpu_ctx *c, int s, int op, double dt, int a, int b) {
@@ -1338,28 +1338,38 @@ extern "C" int sdmd_gpu_last_kernel_ms(sdmd_gpu_ctx *c, double *ms) {
     return SDMD_OK;
 }
 
+// Energy kernels of the context's mode into `planes` (3 x nsys doubles:
+// kinetic, potential, error flag), then the planes into the caller's arrays.
+// No allocation: the error plane is read back through a stack buffer.
+static int energy_launch(sdmd_gpu_ctx *c, double *planes) {
+    if (c->mode == SDMD_MODE_FAST) return sdmd_fast_energy_launch(c, planes);
+    DISPATCH(launch_energy, c, planes);
+    return hip_ok(hipGetLastError());
+}
+
+static int energy_planes_to_host(sdmd_gpu_ctx *c, const double *planes, double *kinetic, double *potential) {
+    const size_t n = (size_t)c->nsys;
+    int rc = hip_ok(hipMemcpyAsync(kinetic, planes, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    if (!rc) rc = hip_ok(hipMemcpyAsync(potential, planes + n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    int bad = 0;
+    for (size_t s0 = 0; !rc && s0 < n; s0 += 64) {
+        double flags[64];
+        const size_t m = n - s0 < 64 ? n - s0 : 64;
+        rc = hip_ok(hipMemcpyAsync(flags, planes + 2 * n + s0, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+        if (!rc) rc = hip_ok(hipStreamSynchronize(c->stream));
+        for (size_t k = 0; !rc && k < m; k++) bad |= flags[k] != 0;
+    }
+    if (!rc) rc = hip_ok(hipStreamSynchronize(c->stream));
+    return rc ? rc : (bad ? SDMD_ESTATE : SDMD_OK);     // a table lookup failed or the system is in error
+}
+
 extern "C" int sdmd_gpu_energy(sdmd_gpu_ctx *c, double *kinetic, double *potential) {
     if (!c || !kinetic || !potential) return SDMD_EINVAL;
     int rc;
-    if (!c->dev_energy) {
-        double *zero = (double *)calloc((size_t)2 * c->nsys, sizeof(double));
-        if (!zero) return SDMD_ENOMEM;
-        rc = dev_upload(c, &c->dev_energy, zero, (size_t)2 * c->nsys);
-        free(zero);
-        if (rc) return rc;
-    }
-    if (c->mode == SDMD_MODE_FAST) return sdmd_fast_energy(c, kinetic, potential);
-    DISPATCH(launch_energy, c);
-    double *h = (double *)malloc(sizeof(double) * 2 * (size_t)c->nsys);
-    if (!h) return SDMD_ENOMEM;
-    rc = hip_ok(hipGetLastError());
-    if (!rc) rc = hip_ok(hipMemcpyAsync(h, c->dev_energy, sizeof(double) * 2 * (size_t)c->nsys, hipMemcpyDeviceToHost,
-                                        c->stream));
-    if (!rc) rc = hip_ok(hipStreamSynchronize(c->stream));
-    if (!rc)
-        for (int s = 0; s < c->nsys; s++) { kinetic[s] = h[2 * s]; potential[s] = h[2 * s + 1]; }
-    free(h);
-    return rc;
+    if (!c->dev_energy && (rc = dev_upload(c, &c->dev_energy, (const double *)nullptr, (size_t)3 * c->nsys)))
+        return rc;
+    if ((rc = energy_launch(c, c->dev_energy))) return rc;
+    return energy_planes_to_host(c, c->dev_energy, kinetic, potential);
 }
 
 extern "C" int sdmd_gpu_set_temperature(sdmd_gpu_ctx *c, int32_t s, double T) {

@@ -89,3 +89,42 @@ def assert_same_schedule_result(runs, min_frames, min_per_round=2):
     assert fb["batches"] * min_per_round < sb["pops"]     # events per round ran concurrently
     for f in STATE_FIELDS:
         assert np.array_equal(a.arr[f], b.arr[f]), f
+
+
+def fast_bulk_vs_oracle(system, block=256, tol_solved=1e-12, tol_inherited=1e-9):
+    """The fast engine's bulk prediction against the oracle's on the same
+    state, with the clock moved to 0 (event times are relative, so the
+    predictions do not change, and the engine's absolute times then carry
+    no rounding of a large clock).  Partner, event type and subtype must be
+    equal for every atom; times within tol_solved (relative) where the
+    oracle solved the pair for the atom itself, within tol_inherited where
+    it copied the partner's time through CheckDuplication
+    (TimePrediction.c:1101-1117): that time was solved from the mirrored
+    pair (r_ji instead of r_ij), whose naive quadratic rounds differently.
+    Returns (max error over solved atoms, max over inherited ones, number
+    of inherited atoms)."""
+    import oracle_lib
+    from sdmd_amd.engine import GpuEngine, System
+
+    arr = system.arrays.copy()
+    arr.scalars["currenttime"] = 0.0
+    sysm = System(None, arr)
+    eng = GpuEngine([sysm], block=block, mode="fast")
+    eng.prepare()
+    st = eng.state(0)
+    eng.close()
+    o = oracle_lib.Oracle(arr)
+    assert o.bulk() == 0
+    ref = o.state().arr
+    inh = o.inherited()[1:]
+    a = st.arr
+    T = st.scalars["currenttime"]          # the engine's snapshot time: its earliest pending event
+    for f in ("ev_partner", "ev_type", "ev_sub"):
+        bad = np.flatnonzero(a[f][1:] != ref[f][1:])
+        assert len(bad) == 0, (f, int(bad[0]) + 1 if len(bad) else None)
+    rel = np.abs((a["ev_time"][1:] + T) - ref["ev_time"][1:]) / ref["ev_time"][1:]
+    e_solved = float(rel[~inh].max()) if (~inh).any() else 0.0
+    e_inh = float(rel[inh].max()) if inh.any() else 0.0
+    assert e_solved <= tol_solved, ("solved", e_solved, int(np.argmax(np.where(inh, 0, rel))) + 1)
+    assert e_inh <= tol_inherited, ("inherited", e_inh)
+    return e_solved, e_inh, int(inh.sum())

@@ -12,8 +12,10 @@ compact .npz fixtures:
   <name>_topo.npz   bonds / constraints / connection map / pair tables
   <name>_trace.npz  every popped event (target, partner, type, subtype,
                     executed flag, dt, potential, RNG position) and the
-                    re-predicted atoms' records for the first pops, plus
-                    FNV state hashes every 1000 pops for the whole run
+                    re-predicted atoms' records for the first pops; target,
+                    partner, type and executed flag of every pop of the run
+                    (seq_*); FNV state hashes every 1000 pops for the whole
+                    run
 
 The fixtures are data (inputs and expected outputs); no reference source
 is copied.
@@ -40,7 +42,8 @@ REF_BIN = os.path.join(ROOT, "oracle", "_ref", "sdmd_ref")
 
 CASES = {
     # name: (builder, parameter overrides, pops, full-record pops)
-    "ala10ext": ("ala10ext", dict(temperature=0.5962, cutoff=9.0), 100000, 5000),
+    # 170 000 pops: more than 10^5 executed events (about 0.6 executed per pop)
+    "ala10ext": ("ala10ext", dict(temperature=0.5962, cutoff=9.0), 170000, 5000),
     "ala10helix": ("helix", dict(temperature=0.5962, cutoff=9.0), 200000, 5000),
     "mix4x20": ("mix", dict(temperature=0.5962, cutoff=9.0), 100000, 3000),
     "ala10nve": ("ala10ext", dict(temperature=0.5962, cutoff=9.0, thermostat="no"), 100000, 2000),
@@ -104,6 +107,8 @@ def save_case(name, dump, full, outdir, inp):
     np.savez_compressed(os.path.join(outdir, name + "_trace.npz"), npops=np.array(len(tr)),
                         target=f["target"].astype(np.int32), partner=f["partner"].astype(np.int32),
                         type=f["type"].astype(np.int8), sub=f["sub"].astype(np.int8),
+                        seq_target=tr["target"].astype(np.int32), seq_partner=tr["partner"].astype(np.int32),
+                        seq_type=tr["type"].astype(np.int8), seq_exec=tr["exec"].astype(np.int8),
                         exec=f["exec"].astype(np.int8), dt=f["dt"], pot=f["pot"],
                         rand_calls=f["rand_calls"], renew=f["r"], nrenew=f["nrenew"],
                         hashes=r["hashes"], counts_type=np.bincount(tr["type"][tr["exec"] == 1], minlength=16),

@@ -44,6 +44,7 @@ def lib():
         L.oracle_rand_calls.argtypes = [P]
         L.oracle_predict.argtypes = [P, C.POINTER(C.c_int32)]
         L.oracle_bulk.argtypes = [P]
+        L.oracle_inherited.argtypes = [P, C.c_void_p]
         L.oracle_create_cbt.argtypes = [P]
         L.oracle_run.restype = C.c_long
         L.oracle_run.argtypes = [P, C.c_long, C.c_void_p, C.c_void_p, C.c_long]
@@ -84,6 +85,13 @@ class Oracle:
 
     def bulk(self):
         return lib().oracle_bulk(self.h)
+
+    def inherited(self):
+        """Boolean per atom (index 0 dummy): the current event record was
+        copied from the partner's by CheckDuplication, not solved."""
+        out = np.zeros(self.arrays.scalars["natom"] + 1, np.int8)
+        lib().oracle_inherited(self.h, out.ctypes.data)
+        return out.astype(bool)
 
     def predict(self, atoms):
         renew = np.zeros(len(atoms) + 2, np.int32)

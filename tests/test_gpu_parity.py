@@ -54,8 +54,10 @@ def test_bulk_predict_matches_reference(require_gpu, name):
 @pytest.mark.parametrize("name", CASES)
 def test_event_sequence_matches_reference(require_gpu, name):
     """Every popped event of the golden run: full records for the stored
-    prefix, FNV hashes of positions/velocities/times/counters every 1000 pops
-    for the whole run (10^5 pops; 2*10^5 for the helix)."""
+    prefix, target/partner/type/executed flag of every pop, FNV hashes of
+    positions/velocities/times/counters every 1000 pops for the whole run
+    (1.7*10^5 pops -- more than 10^5 executed events -- for the ALA10
+    extended chain; 2*10^5 for the helix)."""
     from sdmd_amd.engine import GpuEngine
 
     sysm, g = golden_system(name)
@@ -70,6 +72,10 @@ def test_event_sequence_matches_reference(require_gpu, name):
     assert len(tr) == npops
     mm = first_mismatch(tr, g, nfull)
     assert mm is None, (name, mm)
+    for f in ("target", "partner", "type", "exec"):          # every pop of the run
+        ref_seq = g["trace"]["seq_" + f]
+        bad = np.flatnonzero(tr[f] != ref_seq.astype(tr[f].dtype))
+        assert len(bad) == 0, (name, f, "first mismatch at pop", int(bad[0]) if len(bad) else None)
     hs = eng.hashes(0)
     ref = g["trace"]["hashes"]
     assert hs.shape == ref.shape

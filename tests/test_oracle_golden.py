@@ -40,6 +40,8 @@ def test_oracle_event_sequence(name):
     for f in ["atom", "partner", "type", "sub", "counter", "pcounter", "time", "pot"]:
         a = mine["r"][:n, 0][f][sel]
         assert np.array_equal(a, tr["renew"][:, 0][f][sel].astype(a.dtype)), f
+    for f in ("target", "partner", "type", "exec"):          # every pop of the run
+        assert np.array_equal(mine[f], tr["seq_" + f].astype(mine[f].dtype)), f
     assert np.array_equal(hs, tr["hashes"])
     st = o.state()
     for f in ("x", "y", "z", "vx", "vy", "vz"):
@@ -54,6 +56,15 @@ def test_helix_exercises_hb_events():
     _, g = golden_system("ala10helix")
     counts = g["trace"]["counts_type"]
     assert counts[2] > 100 and counts[3] > 100     # HB and HB-neighbour events
+
+
+def test_config1_golden_run_covers_1e5_events():
+    """The ALA10 extended-chain golden run pins more than 10^5 consecutive
+    executed events of the reference."""
+    _, g = golden_system("ala10ext")
+    tr = g["trace"]
+    assert int(tr["seq_exec"].sum()) >= 100000
+    assert int(tr["counts_type"].sum()) == int(tr["seq_exec"].sum())
 
 
 def test_nve_run_has_no_thermostat_events():
