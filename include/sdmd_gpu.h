@@ -151,6 +151,16 @@ int sdmd_gpu_run_events(sdmd_gpu_ctx *ctx, int64_t events_per_system, double max
  * The sums run in another order than the reference's loop (last-bit
  * differences); the energy that REMD exchanges compare (REMD.c:97). */
 int sdmd_gpu_energy(sdmd_gpu_ctx *ctx, double *kinetic, double *potential);
+/* As sdmd_gpu_energy, but allocating nothing (no device or host memory is
+ * allocated or freed): the per-system partial results live in a fixed
+ * device array of 3 * SDMD_NOALLOC_MAX_SYS doubles that every GPU the library
+ * runs on holds until the process ends.  No counterpart in the reference;
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * REMD exchange loops, allocation-free harnesses).  Returns SDMD_ERANGE for
+ * a context of more than SDMD_NOALLOC_MAX_SYS systems.  Calls on one GPU
+ * share the array: calls that overlap on one GPU race. */
+#define SDMD_NOALLOC_MAX_SYS 4096
+int sdmd_gpu_energy_noalloc(sdmd_gpu_ctx *ctx, double *kinetic, double *potential);
 /* Set one system's thermostat temperature (reduced units, k_B T in
  * kcal/mol) -- `targetTemperature = REMD_Temperature.T` after an exchange,
  * REMD.c:100; the Andersen draws use variance T/m (ToolFunctions.c:605). */

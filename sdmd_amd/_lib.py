@@ -72,7 +72,8 @@ GPU_SYMBOLS = ["sdmd_gpu_open", "sdmd_gpu_close", "sdmd_gpu_set_normals", "sdmd_
                "sdmd_gpu_download", "sdmd_gpu_get_tree", "sdmd_gpu_stats_get", "sdmd_gpu_get_trace",
                "sdmd_gpu_get_hashes", "sdmd_gpu_phase_times", "sdmd_gpu_last_kernel_ms",
                "sdmd_gpu_run_for", "sdmd_gpu_run_until", "sdmd_gpu_fast_stats_get", "sdmd_gpu_autotune",
-               "sdmd_gpu_run_graph", "sdmd_gpu_run_events", "sdmd_gpu_energy", "sdmd_gpu_set_temperature"]
+               "sdmd_gpu_run_graph", "sdmd_gpu_run_events", "sdmd_gpu_energy", "sdmd_gpu_set_temperature",
+               "sdmd_gpu_energy_noalloc"]
 
 
 def lib():
@@ -108,6 +109,7 @@ def lib():
     L.sdmd_gpu_run_graph.argtypes = [P, C.c_int32, C.c_double]
     L.sdmd_gpu_run_events.argtypes = [P, I64, C.c_double]
     L.sdmd_gpu_energy.argtypes = [P, C.c_void_p, C.c_void_p]
+    L.sdmd_gpu_energy_noalloc.argtypes = [P, C.c_void_p, C.c_void_p]
     L.sdmd_gpu_set_temperature.argtypes = [P, I, C.c_double]
     L.sdmd_remd_rearrange.argtypes = [C.c_void_p, C.c_void_p, I, C.c_void_p, C.c_void_p]
     L.sdmd_gpu_fast_stats_get.argtypes = [P, C.c_int32, C.POINTER(FastStats)]

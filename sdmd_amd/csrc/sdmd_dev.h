@@ -727,6 +727,7 @@ struct sdmd_gpu_ctx {
     int nalloc, capalloc;
     int *dev_renew;
     double *dev_energy;        // 3 planes of nsys doubles (kinetic, potential, error flag)
+    double *scratch;           // device address of the allocation-free energy planes on this GPU
     double last_ms;
 };
 

@@ -20,6 +20,8 @@
 // the x86-64 reference build, and double sqrt/div are correctly rounded.
 
 
+#include <mutex>
+
 #include "sdmd_dev.h"
 
 namespace sdmd {
@@ -1020,6 +1022,8 @@ static int sync_from_device(sdmd_gpu_ctx *c) {
     return hip_ok(hipStreamSynchronize(c->stream));
 }
 
+static int resolve_scratch(int dev, double **out);
+
 extern "C" int sdmd_gpu_open(const sdmd_system_desc *const *systems, int32_t nsys, const sdmd_gpu_opts *opts,
                              sdmd_gpu_ctx **out) {
     if (!systems || nsys <= 0 || !out) return SDMD_EINVAL;
@@ -1042,6 +1046,12 @@ extern "C" int sdmd_gpu_open(const sdmd_system_desc *const *systems, int32_t nsy
     c->block = block;
     c->mode = mode;
     if (hipStreamCreate(&c->stream) != hipSuccess) { free(c); return SDMD_EDEVICE; }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || resolve_scratch(dev, &c->scratch) != SDMD_OK) {
+        HIPV(hipStreamDestroy(c->stream));
+        free(c);
+        return SDMD_EDEVICE;
+    }
     HIPV(hipEventCreate(&c->ev0));
     HIPV(hipEventCreate(&c->ev1));
     c->host_sys = (Sys *)calloc((size_t)nsys, sizeof(Sys));
@@ -1370,6 +1380,41 @@ extern "C" int sdmd_gpu_energy(sdmd_gpu_ctx *c, double *kinetic, double *potenti
         return rc;
     if ((rc = energy_launch(c, c->dev_energy))) return rc;
     return energy_planes_to_host(c, c->dev_energy, kinetic, potential);
+}
+
+// ---------------------------------------------------------------- allocation-free energy
+// Per-system partial results of sdmd_gpu_energy_noalloc (three planes of
+// SDMD_NOALLOC_MAX_SYS doubles) in a file-scope device array: every GPU the
+// library runs on holds one copy until the process ends.  The kernels write
+// every entry they report, so nothing needs clearing between calls; calls
+// that overlap on one GPU share the array and race.
+static __device__ __align__(256) double scratch[3 * SDMD_NOALLOC_MAX_SYS];
+
+// the array's device address on each GPU, resolved once per device by
+// sdmd_gpu_open with that device current (host code never uses the
+// array's name as an address)
+constexpr int NOALLOC_MAXDEV = 64;
+static double *g_scratch[NOALLOC_MAXDEV];
+static std::mutex g_scratch_mu;
+
+static int resolve_scratch(int dev, double **out) {
+    if (dev < 0 || dev >= NOALLOC_MAXDEV) return SDMD_EDEVICE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return SDMD_EDEVICE;
+        g_scratch[dev] = (double *)p;
+    }
+    *out = g_scratch[dev];
+    return SDMD_OK;
+}
+
+extern "C" int sdmd_gpu_energy_noalloc(sdmd_gpu_ctx *c, double *kinetic, double *potential) {
+    if (!c || !kinetic || !potential) return SDMD_EINVAL;
+    if (c->nsys > SDMD_NOALLOC_MAX_SYS) return SDMD_ERANGE;
+    if (!c->scratch) return SDMD_EDEVICE;
+    int rc = energy_launch(c, c->scratch);
+    return rc ? rc : energy_planes_to_host(c, c->scratch, kinetic, potential);
 }
 
 extern "C" int sdmd_gpu_set_temperature(sdmd_gpu_ctx *c, int32_t s, double T) {
