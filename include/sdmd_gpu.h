@@ -71,6 +71,9 @@ typedef struct {
                               agreement of the screened prediction with the full FP64 scan */
     int32_t max_batch;     /* fast: at most this many events per round (0 = 64) */
     uint64_t seed;         /* fast: thermostat stream key (0 = the system's seed) */
+    int32_t helpers;       /* fast: helper workgroups per system that take re-predictions on other
+                              CUs (0 = as many as the GPU holds beside the owners, -1 = none) */
+    int32_t pad;
 } sdmd_gpu_opts;
 
 /* one popped event (same layout as the reference trace dump) */
@@ -109,6 +112,7 @@ typedef struct {
     double max_screen_rel_err;   /* largest |t32 - t64| / t64 over confirmed collision winners */
     double max_time_rel_err;     /* largest time error of a disagreeing validated prediction */
     double clock;                /* time of the latest executed event */
+    int64_t helpers;             /* helper workgroups per system of the context's launches */
 } sdmd_gpu_fast_stats;
 
 typedef struct {
@@ -168,9 +172,10 @@ int sdmd_gpu_set_temperature(sdmd_gpu_ctx *ctx, int32_t sys, double temperature)
 int sdmd_gpu_fast_stats_get(sdmd_gpu_ctx *ctx, int32_t sys, sdmd_gpu_fast_stats *out);
 /* Fast mode, batching form: time every workgroup shape (4, 8, 12 or 16
  * wavefronts per system, i.e. re-predictions in flight at once, and 2 or 4
- * candidates staged per lane) for trial_ms of device time on the live
- * systems and keep the fastest; the systems advance meanwhile.  No reference
- * counterpart (the reference is single-threaded). */
+ * candidates staged per lane), then, for the fastest, all, half or none of
+ * the helper workgroups that fit, for trial_ms of device time each on the
+ * live systems, and keep the fastest; the systems advance meanwhile.  No
+ * reference counterpart (the reference is single-threaded). */
 int sdmd_gpu_autotune(sdmd_gpu_ctx *ctx, double trial_ms, int32_t *block_out, int32_t *depth_out);
 /* Fast mode: `launches` event-loop launches of max_ms device time each,
  * captured once into a HIP graph and replayed (re-captured when the shape or

@@ -38,14 +38,16 @@ class RemdTemp(C.Structure):
 class GpuOpts(C.Structure):
     _fields_ = [("mode", C.c_int32), ("block", C.c_int32), ("trace", C.c_int32),
                 ("profile", C.c_int32), ("trace_capacity", C.c_int64), ("hash_every", C.c_int64),
-                ("validate", C.c_int32), ("max_batch", C.c_int32), ("seed", C.c_uint64)]
+                ("validate", C.c_int32), ("max_batch", C.c_int32), ("seed", C.c_uint64),
+                ("helpers", C.c_int32), ("pad", C.c_int32)]
 
 
 class FastStats(C.Structure):
     _fields_ = [("batches", C.c_int64), ("deferred", C.c_int64), ("single", C.c_int64),
                 ("screens", C.c_int64), ("confirms", C.c_int64), ("validated", C.c_int64),
                 ("agree", C.c_int64), ("rebases", C.c_int64), ("causal", C.c_int64),
-                ("max_screen_rel_err", C.c_double), ("max_time_rel_err", C.c_double), ("clock", C.c_double)]
+                ("max_screen_rel_err", C.c_double), ("max_time_rel_err", C.c_double), ("clock", C.c_double),
+                ("helpers", C.c_int64)]
 
 
 class GpuStats(C.Structure):

@@ -60,8 +60,10 @@ constexpr float FINF = 3.0e38f;
 // (the independence rule failed); FERR_DRAIN: a closing drain reached its
 // pop bound with events still pending up to the stop time, so the state is
 // not a snapshot at one time
+// FERR_JOBS: a round's re-predictions were not all completed within
+// FJOB_WAIT ticks (the wait is bounded so the launch always ends)
 enum { FERR_SELECT = 20, FERR_SLOT = 21, FERR_ROW = 22, FERR_CAND = 23, FERR_NBLK = 24, FERR_CAUSAL = 25,
-       FERR_DRAIN = 26 };
+       FERR_DRAIN = 26, FERR_JOBS = 27 };
 
 // phases timed (by wavefront 0) when profiling is on
 enum { FPH_SELECT, FPH_EXEC, FPH_STAGE, FPH_SCREEN, FPH_CONFIRM, FPH_OTHER, FPH_TREE, FPH_EVENT, FPH_BOND,
@@ -1401,6 +1403,75 @@ __device__ __forceinline__ void block_scan2(const FSys &F, int b0, int b1, doubl
     if (b1 >= 0) block_min3(b1, in1, in1 ? e1 : __builtin_inf(), in1 ? p1 : 0, u1, c1, u2, c2, u3);
 }
 
+// ---------------------------------------------------------------- helper workgroups
+// A system's re-predictions of a round can be shared with helper workgroups
+// on other CUs (of the same XCD when workgroups are dispatched round-robin
+// over the XCDs).  The owner workgroup publishes the round's jobs in global
+// memory; every wavefront of the system's workgroups -- the owner's own
+// included -- claims jobs from one word until none is left, and the owner
+// waits for the count of completed jobs.  The owner never needs a helper
+// (its own wavefronts take whatever is unclaimed), so the protocol cannot
+// deadlock on a helper that is not resident.  Agent-scope release/acquire
+// orders the owner's event writes before the helpers' predictions and the
+// helpers' event records before the owner's tree rescan.
+constexpr long long FHELP_IDLE = 1000000000LL;   // ticks (10 s at 100 MHz): a helper with no owner activity leaves
+constexpr long long FJOB_WAIT = 200000000LL;     // ticks (2 s): the owner's bound on one round's predictions
+
+struct __attribute__((aligned(256))) FJobs {
+    unsigned long long word;    // jobs of the round << 32 | next job index
+    int done;                   // jobs of the round completed
+    int stop;                   // the owner has left its loop
+    int err;                    // first error of a helper's prediction
+    int pad0;
+    double Te;                  // FP32-mirror epoch of the round
+    unsigned long long pad[28];
+    int a[2 * FSEL];            // job q: atom and time
+    double t[2 * FSEL];
+};
+
+// helpers' statistics (the owner's live in FSys), summed by the host
+struct FHelp {
+    long long screens, confirms, pair_evals, candidates, validated, agree, causal;
+    double max_screen_err, max_time_err;
+};
+
+__device__ __forceinline__ unsigned long long job_load(unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int int_load(int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima): this wave's
+// vector-memory operations have completed
+__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+// agent-scope release (L2 write-back), waited for before the caller's next
+// store, which publishes what came before
+__device__ __forceinline__ void release_agent() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    wait_stores();
+}
+__device__ __forceinline__ void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+
+// claim the next job of the published round (-1: none left)
+__device__ __forceinline__ int job_claim(FJobs *J) {
+    const unsigned long long old = __hip_atomic_fetch_add(&J->word, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return (unsigned)old < (unsigned)(old >> 32) ? (int)(unsigned)old : -1;
+}
+
+__device__ __forceinline__ void job_done(FJobs *J) {
+    __hip_atomic_fetch_add(&J->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// reset every system's job word before a launch (a kernel, not a memset, so
+// it can sit in a captured graph)
+__global__ void k_fast_jobs_reset(FJobs *jobs, int nsys) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsys) {
+        jobs[s].word = 0;
+        jobs[s].done = 0;
+        jobs[s].stop = 0;
+        jobs[s].err = 0;
+    }
+}
+
 // ---------------------------------------------------------------- shared memory
 struct FShared {
     double b1[FMAXBLK], b2[FMAXBLK], b3[FMAXBLK];
@@ -1513,19 +1584,88 @@ __global__ __launch_bounds__(64 * W) void k_fast_prepare(FSys *systems) {
     fsys_store<W>(&systems[blockIdx.x], Fl);
 }
 
-// the event loop: rounds of concurrent independent events until `npops`
-// pops are done or the device clock passes `budget` ticks (100 MHz)
+// A helper workgroup: its wavefronts claim the owner's published jobs
+// (re-predictions) until the owner stops, then add their statistics to hp.
 template <int W, int FQ>
-__global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long npops, long long budget, double t_end) {
+__device__ void helper_loop(FSys &F, const ScrTab &tab, FShared &sh, FJobs *J, FHelp *hp) {
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    WStats ws = {0, 0, 0, 0};
+    for (;;) {
+        int q = -1, quit = 0;
+        if (lane == 0) {
+            long long idle0 = (long long)wall_clock64();
+            for (;;) {
+                const unsigned long long w = job_load(&J->word);
+                if ((unsigned)w < (unsigned)(w >> 32)) {
+                    q = job_claim(J);
+                    if (q >= 0) break;
+                    continue;
+                }
+                if (int_load(&J->stop)) { quit = 1; break; }
+                if ((long long)wall_clock64() - idle0 > FHELP_IDLE) { quit = 1; break; }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        q = __shfl(q, 0, 64);
+        quit = __shfl(quit, 0, 64);
+        if (quit) break;
+        acquire_agent();                         // the owner's event writes and job list are visible
+        const int a = J->a[q];
+        const double t = J->t[q];
+        if (lane == 0) F.Te = J->Te;             // every wave of the round writes the same epoch
+        int err = 0;
+        predict_fast<FQ>(F, tab, sh.w[wave], a, t, ws, &err);
+        if (lane == 0 && err) atomicCAS(&J->err, 0, err);
+        release_agent();                         // the event record before the completion count
+        if (lane == 0) job_done(J);
+    }
+    if (lane == 0) {
+        atomicAdd((unsigned long long *)&hp->screens, (unsigned long long)ws.screen);
+        atomicAdd((unsigned long long *)&hp->confirms, (unsigned long long)ws.conf);
+        atomicAdd((unsigned long long *)&hp->pair_evals, (unsigned long long)(ws.fp64 + ws.conf));
+        atomicAdd((unsigned long long *)&hp->candidates, (unsigned long long)ws.cand);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        atomicAdd((unsigned long long *)&hp->validated, (unsigned long long)F.validated);
+        atomicAdd((unsigned long long *)&hp->agree, (unsigned long long)F.agree);
+        atomicAdd((unsigned long long *)&hp->causal, (unsigned long long)F.causal);
+        cnt_max(hp->max_screen_err, F.max_screen_err);
+        cnt_max(hp->max_time_err, F.max_time_err);
+    }
+}
+
+// the event loop: rounds of concurrent independent events until `npops`
+// pops are done or the device clock passes `budget` ticks (100 MHz).
+// Workgroup b serves system (b % 8) + 8 * ((b / 8) / (1 + H)) as owner
+// (role 0) or helper (roles 1..H), so with round-robin dispatch over the 8
+// XCDs all workgroups of a system share one XCD's L2.
+template <int W, int FQ>
+__global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, FJobs *jobs, FHelp *help, int nsys, int H,
+                                                     long long npops, long long budget, double t_end) {
     __shared__ FSys Fl;
     __shared__ FShared sh;
     __shared__ ScrTab tab;
-    fsys_load<W>(Fl, &systems[blockIdx.x]);
+    const int sys = (blockIdx.x & 7) + 8 * ((blockIdx.x >> 3) / (H + 1));
+    const int role = (blockIdx.x >> 3) % (H + 1);
+    if (sys >= nsys) return;                     // uniform per workgroup
+    FJobs *J = jobs + sys;
+    fsys_load<W>(Fl, &systems[sys]);
     if (threadIdx.x == 0) Fl.t_stop = t_end;
     tab_load<W>(tab, Fl.scrtab);
     FSys &F = Fl;
     Sys &S = F.s;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    if (role > 0) {
+        if (tid == 0) {                          // statistics of this helper only; no phase clocks
+            S.prof = nullptr;
+            F.validated = F.agree = F.causal = 0;
+            F.max_screen_err = F.max_time_err = 0;
+        }
+        __syncthreads();
+        helper_loop<W, FQ>(F, tab, sh, J, help + sys);
+        return;
+    }
     const int nblk = F.nblk;
     for (int b = tid; b < nblk; b += 64 * W) {
         sh.b1[b] = G(F.b1)[b]; sh.b2[b] = G(F.b2)[b]; sh.b3[b] = G(F.b3)[b];
@@ -1764,14 +1904,61 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
         const int nren = nheavy + nlight;             // <= 2 * nex <= 2 * FSEL: the two ends do not meet
         auto ren_at = [&](int r) { return r < nheavy ? r : 2 * FSEL - nlight + (r - nheavy); };
         err = 0;
-        for (;;) {
-            int r = 0;
-            if (lane == 0) r = atomicAdd(&sh.next_ren, 1);
-            r = __shfl(r, 0, 64);
-            if (r >= nren || sh.err != 0) break;
-            const int q = ren_at(r);
-            predict_fast<FQ>(F, tab, sh.w[wave], sh.ren_a[q], sh.ren_t[q], ws, &err);
-            if (__ballot(err != 0)) break;          // uniform across the wave
+        if (H == 0) {
+            for (;;) {
+                int r = 0;
+                if (lane == 0) r = atomicAdd(&sh.next_ren, 1);
+                r = __shfl(r, 0, 64);
+                if (r >= nren || sh.err != 0) break;
+                const int q = ren_at(r);
+                predict_fast<FQ>(F, tab, sh.w[wave], sh.ren_a[q], sh.ren_t[q], ws, &err);
+                if (__ballot(err != 0)) break;          // uniform across the wave
+            }
+        } else {
+            // publish the round's jobs, then claim them alongside the helpers
+            for (int r = tid; r < nren; r += 64 * W) {
+                const int q = ren_at(r);
+                J->a[r] = sh.ren_a[q];
+                J->t[r] = sh.ren_t[q];
+            }
+            if (tid == 0) {
+                J->Te = F.Te;
+                J->done = 0;
+            }
+            // the barrier does not wait for other wavefronts' stores: each
+            // wave waits for its own (event execution, job list) to complete
+            // in L2, then one agent-scope release writes L2 back
+            wait_stores();
+            __syncthreads();
+            if (tid == 0) {
+                release_agent();
+                __hip_atomic_store(&J->word, (unsigned long long)nren << 32, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __syncthreads();
+            for (;;) {
+                int r = 0;
+                if (lane == 0) r = sh.err != 0 ? -1 : job_claim(J);
+                r = __shfl(r, 0, 64);
+                if (r < 0) break;
+                const int q = ren_at(r);
+                predict_fast<FQ>(F, tab, sh.w[wave], sh.ren_a[q], sh.ren_t[q], ws, &err);
+                if (lane == 0) job_done(J);
+                if (__ballot(err != 0)) break;          // uniform across the wave
+            }
+            if (err) atomicCAS(&sh.err, 0, err);
+            if (tid == 0) {
+                // every claimed job completes (its wavefront is running); an
+                // error ends the wait, and the loop with it
+                const long long w0 = (long long)wall_clock64();
+                while (int_load(&J->done) < nren && *(volatile int *)&sh.err == 0 && int_load(&J->err) == 0) {
+                    if ((long long)wall_clock64() - w0 > FJOB_WAIT) { atomicCAS(&sh.err, 0, FERR_JOBS); break; }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                acquire_agent();                        // the helpers' event records
+                const int he = int_load(&J->err);
+                if (he) atomicCAS(&sh.err, 0, he);
+            }
         }
         for (int r = tid; r < nren; r += 64 * W) {
             const int b = (sh.ren_a[ren_at(r)] - 1) >> 6;
@@ -1816,6 +2003,10 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
         __syncthreads();
         PROF_MARK(S, FPH_TREE, pt);
     }
+    if (H > 0 && tid == 0) {                      // helpers leave once their current job is done
+        release_agent();
+        __hip_atomic_store(&J->stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     for (int b = tid; b < nblk; b += 64 * W) {
         G(F.b1)[b] = sh.b1[b]; G(F.b2)[b] = sh.b2[b]; G(F.b3)[b] = sh.b3[b];
         G(F.ba)[b] = sh.ba[b]; G(F.bb)[b] = sh.bb[b];
@@ -1831,7 +2022,7 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, long long np
         if (sh.err) S.error = sh.err;
         S.currenttime = F.tlast;
     }
-    fsys_store<W>(&systems[blockIdx.x], Fl);
+    fsys_store<W>(&systems[sys], Fl);
 }
 
 }  // namespace sdmd
@@ -1849,7 +2040,25 @@ struct sdmd::FastState {
     hipGraphExec_t gexec;
     int g_launches;
     long long g_budget;
+    int helpers;               // helper workgroups per system (0: owners only)
+    int hmax;                  // most helpers per system the GPU holds beside the owners
+    FJobs *jobs;               // device: per-system job words and lists
+    FHelp *help;               // device: helpers' statistics
+    FHelp *hhelp;              // host copy
 };
+
+// helper workgroups per system that fit beside the owners: one workgroup
+// per CU (the kernel's LDS), systems spread over the 8 XCDs, each system's
+// workgroups on one XCD
+static int helpers_that_fit(int nsys) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+            hipSuccess || cus < 8)
+        return 0;
+    const int per_xcd = (nsys + 7) / 8;
+    const int h = cus / 8 / per_xcd - 1;
+    return h < 0 ? 0 : (h > 15 ? 15 : h);
+}
 
 template <typename T>
 static int fdev_alloc(sdmd_gpu_ctx *c, T **dst, const T *src, size_t n) {
@@ -2015,15 +2224,29 @@ int sdmd_fast_open(sdmd_gpu_ctx *c, const sdmd_gpu_opts *o) {
         if (rc) return rc;
     }
     if ((rc = fdev_alloc(c, &fs->dev, fs->host, (size_t)c->nsys))) return rc;
+    if ((rc = fdev_alloc(c, &fs->jobs, (const FJobs *)nullptr, (size_t)c->nsys))) return rc;
+    if ((rc = fdev_alloc(c, &fs->help, (const FHelp *)nullptr, (size_t)c->nsys))) return rc;
+    fs->hhelp = (FHelp *)calloc((size_t)c->nsys, sizeof(FHelp));
+    if (!fs->hhelp) return SDMD_ENOMEM;
+    fs->hmax = helpers_that_fit(c->nsys);
+    const int want = o ? o->helpers : 0;
+    if (want < -1) return SDMD_EINVAL;
+    fs->helpers = want < 0 ? 0 : (want == 0 ? fs->hmax : (want < fs->hmax ? want : fs->hmax));
     return SDMD_OK;
 }
 
 static int fast_sync_from(sdmd_gpu_ctx *c) {
     FastState *fs = c->fast;
     if (hipMemcpyAsync(fs->host, fs->dev, sizeof(FSys) * c->nsys, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipMemcpyAsync(fs->hhelp, fs->help, sizeof(FHelp) * c->nsys, hipMemcpyDeviceToHost, c->stream) !=
+            hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
         return SDMD_EDEVICE;
-    for (int s = 0; s < c->nsys; s++) c->host_sys[s] = fs->host[s].s;
+    for (int s = 0; s < c->nsys; s++) {
+        c->host_sys[s] = fs->host[s].s;
+        c->host_sys[s].pair_evals += fs->hhelp[s].pair_evals;       // predictions run by helper workgroups
+        c->host_sys[s].candidates += fs->hhelp[s].candidates;
+    }
     return SDMD_OK;
 }
 
@@ -2036,11 +2259,16 @@ int sdmd_fast_sync(sdmd_gpu_ctx *c) { return fast_sync_from(c); }
     case 16: hipLaunchKernelGGL(kern<16>, dim3(c->nsys), dim3(1024), 0, c->stream, __VA_ARGS__); break; \
     default: hipLaunchKernelGGL(kern<1>, dim3(c->nsys), dim3(64), 0, c->stream, __VA_ARGS__); break; } } while (0)
 
-// the event loop for the context's shape (wavefronts x staging depth)
+// the event loop for the context's shape (wavefronts x staging depth x
+// helper workgroups): the job words are reset, then one workgroup per
+// system and role, 8 * ceil(nsys / 8) * (1 + helpers) in all
 static void launch_run(sdmd_gpu_ctx *c, long long pops, long long budget, double t_end) {
     FastState *fs = c->fast;
-    const dim3 g(c->nsys);
-#define FRUN(Wv, Qv) hipLaunchKernelGGL((k_fast_run<Wv, Qv>), g, dim3(64 * Wv), 0, c->stream, fs->dev, pops, budget, t_end)
+    const int H = fs->helpers;
+    hipLaunchKernelGGL(k_fast_jobs_reset, dim3((c->nsys + 255) / 256), dim3(256), 0, c->stream, fs->jobs, c->nsys);
+    const dim3 g(8 * ((c->nsys + 7) / 8) * (1 + H));
+#define FRUN(Wv, Qv) hipLaunchKernelGGL((k_fast_run<Wv, Qv>), g, dim3(64 * Wv), 0, c->stream, fs->dev, fs->jobs, \
+                                        fs->help, c->nsys, H, pops, budget, t_end)
     if (fs->depth == 2) {
         switch (fs->waves) {
         case 4: FRUN(4, 2); break;
@@ -2255,10 +2483,16 @@ int sdmd_fast_stats(sdmd_gpu_ctx *c, int s, sdmd_gpu_fast_stats *out) {
     out->validated = F.validated;
     out->agree = F.agree;
     out->rebases = F.rebases;
-    out->causal = F.causal;
-    out->max_screen_rel_err = F.max_screen_err;
-    out->max_time_rel_err = F.max_time_err;
+    const FHelp &h = c->fast->hhelp[s];          // predictions run by helper workgroups
+    out->screens += h.screens;
+    out->confirms += h.confirms;
+    out->validated += h.validated;
+    out->agree += h.agree;
+    out->causal = F.causal + h.causal;
+    out->max_screen_rel_err = F.max_screen_err > h.max_screen_err ? F.max_screen_err : h.max_screen_err;
+    out->max_time_rel_err = F.max_time_err > h.max_time_err ? F.max_time_err : h.max_time_err;
     out->clock = F.tlast;
+    out->helpers = c->fast->helpers;
     return SDMD_OK;
 }
 
@@ -2269,7 +2503,7 @@ int sdmd_fast_run_graph(sdmd_gpu_ctx *c, int launches, double max_ms) {
     FastState *fs = c->fast;
     long long budget;
     if (launches < 1 || budget_ticks(max_ms, &budget)) return SDMD_EINVAL;
-    const int key = launches * 64 + fs->waves * 4 + fs->depth;
+    const int key = ((launches * 16 + fs->helpers) * 64 + fs->waves) * 8 + fs->depth;
     if (!fs->gexec || fs->g_launches != key || fs->g_budget != budget) {
         if (fs->gexec) { HIPV(hipGraphExecDestroy(fs->gexec)); fs->gexec = nullptr; }
         if (fs->graph) { HIPV(hipGraphDestroy(fs->graph)); fs->graph = nullptr; }
@@ -2323,6 +2557,24 @@ int sdmd_fast_autotune(sdmd_gpu_ctx *c, double trial_ms, int32_t *block_out, int
     }
     fs->waves = bw;
     fs->depth = bq;
+    // helper workgroups for the kept shape: all that fit (the trials above),
+    // half of them, none
+    int bh = fs->helpers;
+    const int hs[2] = {fs->hmax / 2, 0};
+    for (int k = 0; k < 2; k++) {
+        if (hs[k] == bh || (k == 0 && hs[0] == 0)) continue;
+        const int keep = fs->helpers;
+        fs->helpers = hs[k];
+        long long ev0 = 0, ev1 = 0;
+        for (int s = 0; s < c->nsys; s++) ev0 += fs->host[s].s.frame;
+        int rc = sdmd_fast_run(c, 1LL << 60, trial_ms, INFINIT);
+        if (rc) return rc;
+        for (int s = 0; s < c->nsys; s++) ev1 += fs->host[s].s.frame;
+        const double rate = (double)(ev1 - ev0) / (c->last_ms > 0 ? c->last_ms : 1.0);
+        if (rate > best) { best = rate; bh = hs[k]; }
+        fs->helpers = keep;
+    }
+    fs->helpers = bh;
     c->block = 64 * bw;
     if (block_out) *block_out = 64 * bw;
     if (depth_out) *depth_out = bq;
@@ -2334,6 +2586,7 @@ void sdmd_fast_close(sdmd_gpu_ctx *c) {
     if (c->fast->gexec) HIPV(hipGraphExecDestroy(c->fast->gexec));
     if (c->fast->graph) HIPV(hipGraphDestroy(c->fast->graph));
     free(c->fast->host);
+    free(c->fast->hhelp);
     free(c->fast);
     c->fast = nullptr;
 }

@@ -229,3 +229,31 @@ def test_fast_graph_replay_and_autotune_keep_the_trajectory(require_gpu, mid_nvt
             assert np.array_equal(st.arr[f], ref.arr[f]), (kw, f)
         if picked:
             assert picked[0] in (256, 512, 768, 1024) and picked[1] in (2, 4)
+
+
+def test_fast_helper_workgroups_keep_the_trajectory(require_gpu, mid_nvt):
+    """Re-predictions shared with helper workgroups on other CUs (published
+    per round through a job word in global memory, agent-scope
+    release/acquire) reach the same state at the same time, bit for bit, as
+    the owner workgroup alone: a round's predictions read only state the
+    round's events left, whoever runs them."""
+    from sdmd_amd.engine import GpuEngine
+
+    s2, _ = golden_system("mix4x20")
+    runs = []
+    for helpers in (-1, 0, 1):
+        eng = GpuEngine([mid_nvt, s2], block=512, mode="fast", helpers=helpers)
+        eng.prepare()
+        eng.run_until(3.0)
+        fs = [eng.fast_stats(k) for k in range(2)]
+        runs.append(([eng.state(k) for k in range(2)], [eng.stats(k) for k in range(2)], fs))
+        eng.close()
+    assert runs[0][2][0]["helpers"] == 0 and runs[1][2][0]["helpers"] >= 1 and runs[2][2][0]["helpers"] == 1
+    base_states, base_stats, _ = runs[0]
+    for states, stats, fs in runs[1:]:
+        for k in range(2):
+            assert stats[k]["error"] == 0 and fs[k]["causal"] == 0
+            assert stats[k]["frame"] == base_stats[k]["frame"] > 1000
+            assert stats[k]["pair_evals"] == base_stats[k]["pair_evals"]
+            for f in FIELDS:
+                assert np.array_equal(states[k].arr[f], base_states[k].arr[f]), (k, f)
