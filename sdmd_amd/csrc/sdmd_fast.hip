@@ -1405,15 +1405,21 @@ __device__ __forceinline__ void block_scan2(const FSys &F, int b0, int b1, doubl
 
 // ---------------------------------------------------------------- helper workgroups
 // A system's re-predictions of a round can be shared with helper workgroups
-// on other CUs (of the same XCD when workgroups are dispatched round-robin
-// over the XCDs).  The owner workgroup publishes the round's jobs in global
-// memory; every wavefront of the system's workgroups -- the owner's own
-// included -- claims jobs from one word until none is left, and the owner
-// waits for the count of completed jobs.  The owner never needs a helper
-// (its own wavefronts take whatever is unclaimed), so the protocol cannot
-// deadlock on a helper that is not resident.  Agent-scope release/acquire
-// orders the owner's event writes before the helpers' predictions and the
-// helpers' event records before the owner's tree rescan.
+// on other CUs of the owner's XCD.  The owner workgroup publishes the
+// round's jobs in global memory; every wavefront of the system's workgroups
+// -- the owner's own included -- claims jobs from one word until none is
+// left, and the owner waits for the count of completed jobs.  The owner
+// never needs a helper (its own wavefronts take whatever is unclaimed), so
+// the protocol cannot deadlock on a helper that is not resident.
+//
+// Ordering uses the XCD's shared L2: a workgroup reads its XCD from the
+// hardware (XCC_ID), the owner records it in the job record, and a helper
+// takes jobs only when it runs on that XCD.  Writes are published by waiting
+// for them to complete (they write through the CU's L1 into the L2) before
+// the agent-scope atomic that announces them; the reader, after the atomic
+// that observes them, runs an agent-scope acquire (its L1 copies are
+// dropped; a workgroup-scope invalidate does not drop lines other CUs
+// wrote).  No L2 write-back is needed, as no other XCD takes part.
 constexpr long long FHELP_IDLE = 1000000000LL;   // ticks (10 s at 100 MHz): a helper with no owner activity leaves
 constexpr long long FJOB_WAIT = 200000000LL;     // ticks (2 s): the owner's bound on one round's predictions
 
@@ -1422,7 +1428,7 @@ struct __attribute__((aligned(256))) FJobs {
     int done;                   // jobs of the round completed
     int stop;                   // the owner has left its loop
     int err;                    // first error of a helper's prediction
-    int pad0;
+    int xcc;                    // the owner's XCD (-1: helpers stay out)
     double Te;                  // FP32-mirror epoch of the round
     unsigned long long pad[28];
     int a[2 * FSEL];            // job q: atom and time
@@ -1449,6 +1455,8 @@ __device__ __forceinline__ void release_agent() {
     wait_stores();
 }
 __device__ __forceinline__ void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+// the XCD this wavefront runs on (hwreg XCC_ID, bits 3:0)
+__device__ __forceinline__ int xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15; }
 
 // claim the next job of the published round (-1: none left)
 __device__ __forceinline__ int job_claim(FJobs *J) {
@@ -1469,6 +1477,7 @@ __global__ void k_fast_jobs_reset(FJobs *jobs, int nsys) {
         jobs[s].done = 0;
         jobs[s].stop = 0;
         jobs[s].err = 0;
+        jobs[s].xcc = -1;
     }
 }
 
@@ -1587,7 +1596,7 @@ __global__ __launch_bounds__(64 * W) void k_fast_prepare(FSys *systems) {
 // A helper workgroup: its wavefronts claim the owner's published jobs
 // (re-predictions) until the owner stops, then add their statistics to hp.
 template <int W, int FQ>
-__device__ void helper_loop(FSys &F, const ScrTab &tab, FShared &sh, FJobs *J, FHelp *hp) {
+__device__ void helper_loop(FSys &F, const ScrTab &tab, FShared &sh, FJobs *J, FHelp *hp, int xcc) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     WStats ws = {0, 0, 0, 0};
     for (;;) {
@@ -1597,6 +1606,7 @@ __device__ void helper_loop(FSys &F, const ScrTab &tab, FShared &sh, FJobs *J, F
             for (;;) {
                 const unsigned long long w = job_load(&J->word);
                 if ((unsigned)w < (unsigned)(w >> 32)) {
+                    if (int_load(&J->xcc) != xcc) { quit = 1; break; }   // not the owner's XCD
                     q = job_claim(J);
                     if (q >= 0) break;
                     continue;
@@ -1609,14 +1619,14 @@ __device__ void helper_loop(FSys &F, const ScrTab &tab, FShared &sh, FJobs *J, F
         q = __shfl(q, 0, 64);
         quit = __shfl(quit, 0, 64);
         if (quit) break;
-        acquire_agent();                         // the owner's event writes and job list are visible
+        acquire_agent();                         // drop stale L1 lines: the owner's writes, from the L2
         const int a = J->a[q];
         const double t = J->t[q];
         if (lane == 0) F.Te = J->Te;             // every wave of the round writes the same epoch
         int err = 0;
         predict_fast<FQ>(F, tab, sh.w[wave], a, t, ws, &err);
         if (lane == 0 && err) atomicCAS(&J->err, 0, err);
-        release_agent();                         // the event record before the completion count
+        wait_stores();                           // the event record is in the L2 before the count
         if (lane == 0) job_done(J);
     }
     if (lane == 0) {
@@ -1656,6 +1666,7 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, FJobs *jobs,
     FSys &F = Fl;
     Sys &S = F.s;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int xcc = xcc_id();
     if (role > 0) {
         if (tid == 0) {                          // statistics of this helper only; no phase clocks
             S.prof = nullptr;
@@ -1663,8 +1674,12 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, FJobs *jobs,
             F.max_screen_err = F.max_time_err = 0;
         }
         __syncthreads();
-        helper_loop<W, FQ>(F, tab, sh, J, help + sys);
+        helper_loop<W, FQ>(F, tab, sh, J, help + sys, xcc);
         return;
+    }
+    if (H > 0 && tid == 0) {                     // helpers on this XCD may join
+        __hip_atomic_store(&J->xcc, xcc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        wait_stores();
     }
     const int nblk = F.nblk;
     for (int b = tid; b < nblk; b += 64 * W) {
@@ -1927,14 +1942,12 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, FJobs *jobs,
             }
             // the barrier does not wait for other wavefronts' stores: each
             // wave waits for its own (event execution, job list) to complete
-            // in L2, then one agent-scope release writes L2 back
+            // in the L2 before the word announces the round
             wait_stores();
             __syncthreads();
-            if (tid == 0) {
-                release_agent();
+            if (tid == 0)
                 __hip_atomic_store(&J->word, (unsigned long long)nren << 32, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
-            }
             __syncthreads();
             for (;;) {
                 int r = 0;
@@ -1955,7 +1968,7 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, FJobs *jobs,
                     if ((long long)wall_clock64() - w0 > FJOB_WAIT) { atomicCAS(&sh.err, 0, FERR_JOBS); break; }
                     __builtin_amdgcn_s_sleep(1);
                 }
-                acquire_agent();                        // the helpers' event records
+                acquire_agent();                        // the helpers' event records, from the L2
                 const int he = int_load(&J->err);
                 if (he) atomicCAS(&sh.err, 0, he);
             }
@@ -2003,10 +2016,8 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, FJobs *jobs,
         __syncthreads();
         PROF_MARK(S, FPH_TREE, pt);
     }
-    if (H > 0 && tid == 0) {                      // helpers leave once their current job is done
-        release_agent();
+    if (H > 0 && tid == 0)                        // helpers leave once their current job is done
         __hip_atomic_store(&J->stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
     for (int b = tid; b < nblk; b += 64 * W) {
         G(F.b1)[b] = sh.b1[b]; G(F.b2)[b] = sh.b2[b]; G(F.b3)[b] = sh.b3[b];
         G(F.ba)[b] = sh.ba[b]; G(F.bb)[b] = sh.bb[b];
@@ -2307,6 +2318,16 @@ static int fast_errors(sdmd_gpu_ctx *c) {
         if (c->fast->host[s].s.error) return SDMD_ESTATE;
     return SDMD_OK;
 }
+
+// one window of `events` more executed events per system, enqueued on the
+// context's stream without waiting (the device REMD loop)
+int sdmd_fast_launch_events(sdmd_gpu_ctx *c, long long events) {
+    launch_run(c, -events, 1LL << 62, INFINIT);
+    return hipGetLastError() == hipSuccess ? SDMD_OK : SDMD_EDEVICE;
+}
+
+// device address of system s's thermostat temperature
+double *sdmd_fast_temperature_addr(sdmd_gpu_ctx *c, int s) { return &c->fast->dev[s].s.T; }
 
 int sdmd_fast_prepare(sdmd_gpu_ctx *c) {
     FastState *fs = c->fast;
