@@ -50,7 +50,10 @@
 extern "C" {
 #endif
 
+#ifndef SDMD_GPU_CTX_TYPEDEF
+#define SDMD_GPU_CTX_TYPEDEF
 typedef struct sdmd_gpu_ctx sdmd_gpu_ctx;
+#endif
 
 enum {
     SDMD_MODE_PLAIN = 0,   /* FP64 on the reference's swept state: bit-exact with the reference */
@@ -95,7 +98,7 @@ typedef struct {
     int64_t normals_used;              /* thermostat stream position */
     double currenttime;
     int32_t error;                     /* device error code (0 = none) */
-    int32_t pad;
+    int32_t last_exec;                 /* plain: the last pop executed its event (not stale) */
 } sdmd_gpu_stats;
 
 /* fast-mode counters of one system */
@@ -143,6 +146,15 @@ int sdmd_gpu_run_for(sdmd_gpu_ctx *ctx, int64_t max_pops_per_system, double max_
  * t_end; fast mode executes every event with time <= t_end), at most
  * max_pops pops per system. */
 int sdmd_gpu_run_until(sdmd_gpu_ctx *ctx, double t_end, int64_t max_pops_per_system);
+/* Plain mode: pop while the clock is at most t_end (the loop bound of
+ * SingleThreadRun, SGThread.c:75) and stop after the first executed event
+ * (stale pops do not count) that leaves the clock at or past t_out -- the
+ * point where SingleThreadRun writes its output files, `currenttime >=
+ * outputrecord` after DoEvent, Predict and UpdateCBT (SGThread.c:130-139);
+ * at most max_pops pops per system.  The stats' currenttime and last_exec
+ * tell which bound ended the run.  Fast mode: SDMD_EUNSUPPORTED (use
+ * sdmd_gpu_run_until). */
+int sdmd_gpu_run_to_output(sdmd_gpu_ctx *ctx, double t_out, double t_end, int64_t max_pops_per_system);
 /* Run until every system has executed `events` more events (stale pops do
  * not count; fast mode finishes with its closing drain, so a system may
  * execute a few more), or max_ms of device time has passed (<= 0: no limit).
@@ -194,6 +206,13 @@ int sdmd_gpu_cell_move(sdmd_gpu_ctx *ctx, int32_t sys, int32_t atom);
  * sizes as the desc it was opened with). */
 int sdmd_gpu_download(sdmd_gpu_ctx *ctx, int32_t sys, sdmd_system_desc *out);
 int sdmd_gpu_get_tree(sdmd_gpu_ctx *ctx, int32_t sys, int32_t *node, int32_t *leaf);
+/* Plain mode: replace one system's event tree (node[1..2N-1], leaf[1..N],
+ * as sdmd_gpu_get_tree returns them) -- a resumed run continues with the
+ * tree it was saved with, since after TimeForward rounding the tree's tie
+ * order depends on its history (CBT.c:56-83).  No reference counterpart:
+ * the reference's continue path rebuilds the tree (CreateCBT,
+ * SGThread.c:54-62). */
+int sdmd_gpu_set_tree(sdmd_gpu_ctx *ctx, int32_t sys, const int32_t *node, const int32_t *leaf);
 int sdmd_gpu_stats_get(sdmd_gpu_ctx *ctx, int32_t sys, sdmd_gpu_stats *out);
 /* Trace of the last runs (when opts.trace): copies up to max records. */
 int sdmd_gpu_get_trace(sdmd_gpu_ctx *ctx, int32_t sys, sdmd_gpu_pop_rec *out, int64_t max, int64_t *n);

@@ -43,6 +43,26 @@ typedef struct {
  * (Server.c:162, 172).  Returns 0, or SDMD_EINVAL for bad arguments. */
 int sdmd_remd_rearrange(const double *epot, sdmd_remd_temp *T, int32_t n, sdmd_rand_state *rng, int64_t *counts);
 
+#ifndef SDMD_GPU_CTX_TYPEDEF
+#define SDMD_GPU_CTX_TYPEDEF
+typedef struct sdmd_gpu_ctx sdmd_gpu_ctx;
+#endif
+
+/* The REMD loop on the device, the context's systems being the replicas
+ * (REMDRun's exchange cadence, REMD.c:97-104, with the server's pass,
+ * Server.c:145-192): `cycles` times, every system executes `interval` more
+ * events, the device computes every replica's CalPotenE and runs the pass
+ * above on it with the rand() stream `rng` kept on the device, and writes
+ * each replica's new thermostat temperature -- one host synchronisation for
+ * all cycles.  temps / rng / counts as sdmd_remd_rearrange (in: the state
+ * before, out: after the last pass; counts may be NULL).  history (cycles *
+ * n entries, may be NULL) receives the table after each pass, epot_history
+ * (cycles * n, may be NULL) the energies each pass used.  The acceptance
+ * test uses the device's exp(), which may differ from the C library's in
+ * the last bit. */
+int sdmd_remd_run(sdmd_gpu_ctx *ctx, sdmd_remd_temp *temps, sdmd_rand_state *rng, int64_t *counts, int32_t cycles,
+                  int64_t interval, sdmd_remd_temp *history, double *epot_history);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,21 @@ class GpuOpts(C.Structure):
                 ("helpers", C.c_int32), ("pad", C.c_int32)]
 
 
+class DatInfo(C.Structure):
+    """sdmd_dat_info (include/sdmd_io.h)"""
+    _fields_ = [("frame", C.c_int64), ("event_sums", C.c_int64 * 7), ("timestep", C.c_double),
+                ("outputrate", C.c_double), ("hb_formed", C.c_int32), ("alpha_hb_formed", C.c_int32),
+                ("remd_flag", C.c_int32), ("remd_port", C.c_int32), ("remd_output_rate", C.c_int32),
+                ("remd_server", C.c_char * 50), ("remd_extra", C.c_char * 50), ("remd_num", C.c_int32),
+                ("remd_temperature", C.c_double)]
+
+
+class Resume(C.Structure):
+    """sdmd_resume (include/sdmd_io.h)"""
+    _fields_ = [("natom", C.c_int32), ("seed", C.c_uint32), ("rand_calls", C.c_int64), ("pops", C.c_int64),
+                ("recal", C.c_int64), ("node", C.c_void_p), ("leaf", C.c_void_p)]
+
+
 class FastStats(C.Structure):
     _fields_ = [("batches", C.c_int64), ("deferred", C.c_int64), ("single", C.c_int64),
                 ("screens", C.c_int64), ("confirms", C.c_int64), ("validated", C.c_int64),
@@ -54,7 +69,7 @@ class GpuStats(C.Structure):
     _fields_ = [("pops", C.c_int64), ("frame", C.c_int64), ("recal", C.c_int64),
                 ("per_type", C.c_int64 * 16), ("pair_evals", C.c_int64), ("candidates", C.c_int64),
                 ("normals_used", C.c_int64), ("currenttime", C.c_double), ("error", C.c_int32),
-                ("pad", C.c_int32)]
+                ("last_exec", C.c_int32)]
 
 
 class GpuEvent(C.Structure):
@@ -67,7 +82,9 @@ class GpuEvent(C.Structure):
 HOST_SYMBOLS = ["sdmd_host_load", "sdmd_host_prepare", "sdmd_host_desc", "sdmd_host_normals",
                 "sdmd_normal_stream", "sdmd_host_energy",
                 "sdmd_rand_seed", "sdmd_rand_next", "sdmd_host_free", "sdmd_strerror",
-                "sdmd_remd_rearrange"]
+                "sdmd_remd_rearrange", "sdmd_dat_info_init", "sdmd_write_gro", "sdmd_write_cnt",
+                "sdmd_save_dat", "sdmd_host_load_dat", "sdmd_save_resume", "sdmd_load_resume",
+                "sdmd_remd_run"]
 GPU_SYMBOLS = ["sdmd_gpu_open", "sdmd_gpu_close", "sdmd_gpu_set_normals", "sdmd_gpu_prepare",
                "sdmd_gpu_run", "sdmd_gpu_predict", "sdmd_gpu_create_tree", "sdmd_gpu_update_tree",
                "sdmd_gpu_pop", "sdmd_gpu_advance", "sdmd_gpu_do_event", "sdmd_gpu_cell_move",
@@ -75,7 +92,7 @@ GPU_SYMBOLS = ["sdmd_gpu_open", "sdmd_gpu_close", "sdmd_gpu_set_normals", "sdmd_
                "sdmd_gpu_get_hashes", "sdmd_gpu_phase_times", "sdmd_gpu_last_kernel_ms",
                "sdmd_gpu_run_for", "sdmd_gpu_run_until", "sdmd_gpu_fast_stats_get", "sdmd_gpu_autotune",
                "sdmd_gpu_run_graph", "sdmd_gpu_run_events", "sdmd_gpu_energy", "sdmd_gpu_set_temperature",
-               "sdmd_gpu_energy_noalloc"]
+               "sdmd_gpu_energy_noalloc", "sdmd_gpu_run_to_output", "sdmd_gpu_set_tree"]
 
 
 def lib():
@@ -94,6 +111,14 @@ def lib():
     L.sdmd_host_normals.argtypes = [P, C.c_void_p, C.c_long, C.c_void_p]
     L.sdmd_normal_stream.argtypes = [C.c_uint32, C.c_int64, C.c_void_p, C.c_long, C.c_void_p]
     L.sdmd_host_energy.argtypes = [C.POINTER(SystemDesc), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.sdmd_dat_info_init.restype = None
+    L.sdmd_dat_info_init.argtypes = [C.POINTER(DatInfo), P, C.POINTER(SystemDesc)]
+    L.sdmd_write_gro.argtypes = [P, C.POINTER(SystemDesc), C.c_char_p, C.c_int32]
+    L.sdmd_write_cnt.argtypes = [C.POINTER(SystemDesc), C.c_char_p, C.c_int32]
+    L.sdmd_save_dat.argtypes = [P, C.POINTER(SystemDesc), C.POINTER(DatInfo), C.c_char_p]
+    L.sdmd_host_load_dat.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(P), C.POINTER(DatInfo), C.c_char_p, C.c_int32]
+    L.sdmd_save_resume.argtypes = [C.POINTER(Resume), C.c_char_p]
+    L.sdmd_load_resume.argtypes = [C.POINTER(Resume), C.c_char_p]
     L.sdmd_rand_seed.argtypes = [C.c_void_p, C.c_uint32]
     L.sdmd_rand_next.restype = C.c_int32
     L.sdmd_rand_next.argtypes = [C.c_void_p]
@@ -112,8 +137,11 @@ def lib():
     L.sdmd_gpu_run_events.argtypes = [P, I64, C.c_double]
     L.sdmd_gpu_energy.argtypes = [P, C.c_void_p, C.c_void_p]
     L.sdmd_gpu_energy_noalloc.argtypes = [P, C.c_void_p, C.c_void_p]
+    L.sdmd_gpu_run_to_output.argtypes = [P, C.c_double, C.c_double, I64]
+    L.sdmd_gpu_set_tree.argtypes = [P, C.c_int32, C.c_void_p, C.c_void_p]
     L.sdmd_gpu_set_temperature.argtypes = [P, I, C.c_double]
     L.sdmd_remd_rearrange.argtypes = [C.c_void_p, C.c_void_p, I, C.c_void_p, C.c_void_p]
+    L.sdmd_remd_run.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, I64, C.c_void_p, C.c_void_p]
     L.sdmd_gpu_fast_stats_get.argtypes = [P, C.c_int32, C.POINTER(FastStats)]
     L.sdmd_gpu_predict.argtypes = [P, I, C.c_void_p]
     L.sdmd_gpu_create_tree.argtypes = [P, I]

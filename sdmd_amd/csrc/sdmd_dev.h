@@ -87,6 +87,7 @@ struct Sys {
     long long hash_every, hash_cap, hash_n;
     unsigned long long *prof;   // per-phase wall-clock ticks, or null
     int steps_sorted;           // every step list strictly increasing and non-zero
+    int last_exec;              // the last pop executed its event
 };
 
 // phases timed when a context is opened with profiling on
@@ -763,5 +764,7 @@ int sdmd_fast_sync(sdmd_gpu_ctx *c);
 int sdmd_fast_run_graph(sdmd_gpu_ctx *c, int launches, double max_ms);
 int sdmd_fast_autotune(sdmd_gpu_ctx *c, double trial_ms, int32_t *block_out, int32_t *depth_out);
 int sdmd_fast_energy_launch(sdmd_gpu_ctx *c, double *planes);
+int sdmd_fast_launch_events(sdmd_gpu_ctx *c, long long events);
+double *sdmd_fast_temperature_addr(sdmd_gpu_ctx *c, int s);
 int sdmd_fast_set_temperature(sdmd_gpu_ctx *c, int s, double T);
 void sdmd_fast_close(sdmd_gpu_ctx *c);

@@ -23,6 +23,7 @@
 #include <mutex>
 
 #include "sdmd_dev.h"
+#include "sdmd_remd.h"
 
 namespace sdmd {
 
@@ -752,7 +753,10 @@ __global__ __launch_bounds__(B) void k_prepare(Sys *systems) {
 }
 
 template <int B>
-__global__ __launch_bounds__(B) void k_run(Sys *systems, long long npops, long long budget, double t_end) {
+// t_out: stop after the first executed event that leaves the clock at or
+// past t_out -- where SingleThreadRun writes its output (SGThread.c:130-139)
+__global__ __launch_bounds__(B) void k_run(Sys *systems, long long npops, long long budget, double t_end,
+                                           double t_out) {
     __shared__ Smem sm;
     __shared__ Sys Sl;
     __shared__ int stop;
@@ -823,6 +827,8 @@ __global__ __launch_bounds__(B) void k_run(Sys *systems, long long npops, long l
             for (int k = 1; sm.renew[k]; k++) cbt_update_run(S, sm.renew[k]);
             PROF_MARK(S, PH_TREE, pt);
             S.pops++;
+            S.last_exec = !sm.stale;
+            if (!sm.stale && S.currenttime >= t_out) stop = 1;
             if (S.trace && S.trace_n < S.trace_cap) {
                 sdmd_gpu_pop_rec &r = S.trace[S.trace_n];
                 r.exec = !sm.stale;
@@ -1105,8 +1111,8 @@ static void launch_prepare(sdmd_gpu_ctx *c) {
     hipLaunchKernelGGL((k_prepare<B>), dim3(c->nsys), dim3(B), 0, c->stream, c->dev_sys);
 }
 template <int B>
-static void launch_run(sdmd_gpu_ctx *c, long long n, long long budget, double t_end) {
-    hipLaunchKernelGGL((k_run<B>), dim3(c->nsys), dim3(B), 0, c->stream, c->dev_sys, n, budget, t_end);
+static void launch_run(sdmd_gpu_ctx *c, long long n, long long budget, double t_end, double t_out) {
+    hipLaunchKernelGGL((k_run<B>), dim3(c->nsys), dim3(B), 0, c->stream, c->dev_sys, n, budget, t_end, t_out);
 }
 template <int B>
 static void launch_energy(sdmd_gpu_ctx *c, double *planes) {
@@ -1144,7 +1150,7 @@ extern "C" int sdmd_gpu_prepare(sdmd_gpu_ctx *c) {
     return check_errors(c);
 }
 
-static int run_bounded(sdmd_gpu_ctx *c, int64_t pops, double max_ms, double t_end);
+static int run_bounded(sdmd_gpu_ctx *c, int64_t pops, double max_ms, double t_end, double t_out = INFINITY);
 
 extern "C" int sdmd_gpu_run(sdmd_gpu_ctx *c, int64_t pops) {
     return pops < 0 ? SDMD_EINVAL : run_bounded(c, pops, 0, INFINIT);
@@ -1158,13 +1164,19 @@ extern "C" int sdmd_gpu_run_until(sdmd_gpu_ctx *c, double t_end, int64_t max_pop
     return max_pops < 0 ? SDMD_EINVAL : run_bounded(c, max_pops, 0, t_end);
 }
 
+extern "C" int sdmd_gpu_run_to_output(sdmd_gpu_ctx *c, double t_out, double t_end, int64_t max_pops) {
+    if (!c || max_pops < 0) return SDMD_EINVAL;
+    if (c->mode != SDMD_MODE_PLAIN) return SDMD_EUNSUPPORTED;
+    return run_bounded(c, max_pops, 0, t_end, t_out);
+}
+
 extern "C" int sdmd_gpu_run_events(sdmd_gpu_ctx *c, int64_t events, double max_ms) {
     if (!c || events <= 0) return SDMD_EINVAL;
     return run_bounded(c, -events, max_ms, INFINIT);
 }
 
 // pops >= 0: pops per system; pops < 0: -pops executed events per system
-static int run_bounded(sdmd_gpu_ctx *c, int64_t pops, double max_ms, double t_end) {
+static int run_bounded(sdmd_gpu_ctx *c, int64_t pops, double max_ms, double t_end, double t_out) {
     if (!c) return SDMD_EINVAL;
     if (c->mode == SDMD_MODE_FAST) return sdmd_fast_run(c, pops, max_ms, t_end);
     long long budget = 1LL << 62;
@@ -1176,7 +1188,7 @@ static int run_bounded(sdmd_gpu_ctx *c, int64_t pops, double max_ms, double t_en
         budget = (long long)(max_ms * khz);
     }
     HIPV(hipEventRecord(c->ev0, c->stream));
-    DISPATCH(launch_run, c, (long long)pops, budget, t_end);
+    DISPATCH(launch_run, c, (long long)pops, budget, t_end, t_out);
     HIPV(hipEventRecord(c->ev1, c->stream));
     if (hipGetLastError() != hipSuccess) return SDMD_EDEVICE;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return SDMD_EDEVICE;
@@ -1277,6 +1289,23 @@ extern "C" int sdmd_gpu_download(sdmd_gpu_ctx *c, int32_t s, sdmd_system_desc *d
     return SDMD_OK;
 }
 
+extern "C" int sdmd_gpu_set_tree(sdmd_gpu_ctx *c, int32_t s, const int32_t *node, const int32_t *leaf) {
+    if (!c || s < 0 || s >= c->nsys || !node || !leaf) return SDMD_EINVAL;
+    if (c->mode != SDMD_MODE_PLAIN) return SDMD_EUNSUPPORTED;
+    const int N = c->host_sys[s].N;
+    for (int a = 1; a <= N; a++)                       // every atom's leaf holds it
+        if (leaf[a] < 1 || leaf[a] >= 2 * N || node[leaf[a]] != a) return SDMD_EINVAL;
+    int rc = sync_from_device(c);
+    if (rc) return rc;
+    Sys &S = c->host_sys[s];
+    S.count = N;                                       // every atom inserted (CreateCBT's state)
+    if (hipMemcpy(S.node, node, sizeof(int) * 2 * (size_t)N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(S.leaf, leaf, sizeof(int) * ((size_t)N + 1), hipMemcpyHostToDevice) != hipSuccess)
+        return SDMD_EDEVICE;
+    rc = sync_to_device(c);
+    return rc ? rc : hip_ok(hipStreamSynchronize(c->stream));
+}
+
 extern "C" int sdmd_gpu_get_tree(sdmd_gpu_ctx *c, int32_t s, int32_t *node, int32_t *leaf) {
     if (!c || s < 0 || s >= c->nsys) return SDMD_EINVAL;
     if (c->mode != SDMD_MODE_PLAIN) return SDMD_EUNSUPPORTED;
@@ -1298,6 +1327,7 @@ extern "C" int sdmd_gpu_stats_get(sdmd_gpu_ctx *c, int32_t s, sdmd_gpu_stats *ou
     out->normals_used = S.zpos;
     out->currenttime = S.currenttime;
     out->error = S.error;
+    out->last_exec = S.last_exec;
     return SDMD_OK;
 }
 
@@ -1380,6 +1410,137 @@ extern "C" int sdmd_gpu_energy(sdmd_gpu_ctx *c, double *kinetic, double *potenti
         return rc;
     if ((rc = energy_launch(c, c->dev_energy))) return rc;
     return energy_planes_to_host(c, c->dev_energy, kinetic, potential);
+}
+
+// ---------------------------------------------------------------- replica exchange on the device
+// The REMD server's exchange pass (Server.c:145-192, restated on the host by
+// sdmd_remd_rearrange) run by one device thread between event windows, on
+// the energies the energy kernel left in its planes, with the server's
+// rand() stream (glibc TYPE_3, as sdmd_rand_next) kept in device memory;
+// each replica's thermostat temperature is then written where the event
+// loop reads it.  The pass is sequential by definition (later replicas see
+// earlier swaps, draws happen in order), so one thread runs it.
+__device__ int32_t remd_rand_next(sdmd_rand_state &st) {
+    const uint32_t v = (uint32_t)st.r[st.f] + (uint32_t)st.r[st.b];
+    st.r[st.f] = (int32_t)v;
+    if (++st.f >= 31) st.f = 0;
+    if (++st.b >= 31) st.b = 0;
+    st.calls++;
+    return (int32_t)(v >> 1);
+}
+
+__global__ void k_remd_exchange(const double *planes, int n, sdmd_remd_temp *T, sdmd_rand_state *rng,
+                                long long *counts, double *const *tptr, sdmd_remd_temp *hist, double *ehist,
+                                int *err) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double *epot = planes + n;                 // the potential plane
+    for (int k = 0; k < n; k++)
+        if (planes[2 * n + k] != 0) *err = 1;         // an energy kernel error: no exchange
+    if (*err) return;
+    sdmd_rand_state st = *rng;
+    for (int i = 0; i < n; i++) {
+        if (!T[i].T) continue;
+        for (int j = 0; j < n; j++) {
+            if (!T[j].T) continue;
+            if (T[i].num == T[j].num + 1 || (T[i].num == 1 && T[j].num == 2)) {
+                const double beta1 = 1 / T[i].T, beta2 = 1 / T[j].T;
+                const double dE = (beta1 - beta2) * (epot[j] - epot[i]);
+                const int lo = T[i].num < T[j].num ? T[i].num : T[j].num;
+                if (lo >= 1 && lo <= n) counts[2 * (lo - 1)]++;
+                bool swap = dE <= 0.0;
+                if (!swap) {
+                    const double u = 0 + (double)remd_rand_next(st) / ((double)2147483647 / (1 - 0));   // RANDOMVALUE(0, 1)
+                    swap = u < exp(-1 * dE);
+                }
+                if (swap) {
+                    const sdmd_remd_temp z = T[i];
+                    T[i] = T[j];
+                    T[j] = z;
+                    if (lo >= 1 && lo <= n) counts[2 * (lo - 1) + 1]++;
+                }
+                break;
+            }
+        }
+    }
+    *rng = st;
+    for (int k = 0; k < n; k++) {
+        *tptr[k] = T[k].T;                             // targetTemperature = REMD_Temperature.T, REMD.c:100
+        if (hist) hist[k] = T[k];
+        if (ehist) ehist[k] = epot[k];
+    }
+}
+
+extern "C" int sdmd_remd_run(sdmd_gpu_ctx *c, sdmd_remd_temp *temps, sdmd_rand_state *rng, int64_t *counts,
+                             int32_t cycles, int64_t interval, sdmd_remd_temp *history, double *epot_history) {
+    if (!c || !temps || !rng || cycles < 1 || interval < 1) return SDMD_EINVAL;
+    const int n = c->nsys;
+    for (int k = 0; k < n; k++)
+        if (!(temps[k].T >= 0)) return SDMD_EINVAL;
+    int rc;
+    if (!c->dev_energy && (rc = dev_upload(c, &c->dev_energy, (const double *)nullptr, (size_t)3 * n))) return rc;
+    // device copies for this call (freed at its end)
+    sdmd_remd_temp *dT = nullptr, *dH = nullptr;
+    sdmd_rand_state *dR = nullptr;
+    long long *dC = nullptr;
+    double **dP = nullptr, *dE = nullptr;
+    int *dErr = nullptr;
+    double **tp = (double **)malloc(sizeof(double *) * n);
+    if (!tp) return SDMD_ENOMEM;
+    for (int k = 0; k < n; k++) tp[k] = c->mode == SDMD_MODE_FAST ? sdmd_fast_temperature_addr(c, k) : &c->dev_sys[k].T;
+    rc = SDMD_OK;
+    const size_t hn = (size_t)cycles * n;
+    if (hipMalloc(&dT, sizeof(sdmd_remd_temp) * n) != hipSuccess || hipMalloc(&dR, sizeof(sdmd_rand_state)) != hipSuccess ||
+        hipMalloc(&dC, sizeof(long long) * 2 * n) != hipSuccess || hipMalloc(&dP, sizeof(double *) * n) != hipSuccess ||
+        hipMalloc(&dErr, sizeof(int)) != hipSuccess ||
+        (history && hipMalloc(&dH, sizeof(sdmd_remd_temp) * hn) != hipSuccess) ||
+        (epot_history && hipMalloc(&dE, sizeof(double) * hn) != hipSuccess))
+        rc = SDMD_EDEVICE;
+    long long *zero = (long long *)calloc((size_t)2 * n, sizeof(long long));
+    if (!rc && !zero) rc = SDMD_ENOMEM;
+    if (!rc) rc = hip_ok(hipMemcpy(dT, temps, sizeof(sdmd_remd_temp) * n, hipMemcpyHostToDevice));
+    if (!rc) rc = hip_ok(hipMemcpy(dR, rng, sizeof(sdmd_rand_state), hipMemcpyHostToDevice));
+    if (!rc) rc = hip_ok(hipMemcpy(dC, counts ? (const void *)counts : (const void *)zero, sizeof(long long) * 2 * n,
+                                   hipMemcpyHostToDevice));
+    if (!rc) rc = hip_ok(hipMemcpy(dP, tp, sizeof(double *) * n, hipMemcpyHostToDevice));
+    if (!rc) rc = hip_ok(hipMemset(dErr, 0, sizeof(int)));
+    // the temperatures the table holds now, before the first window
+    if (!rc) {
+        for (int k = 0; k < n && !rc; k++)
+            rc = hip_ok(hipMemcpy(tp[k], &temps[k].T, sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (!rc && c->mode == SDMD_MODE_PLAIN) rc = sync_from_device(c);    // host mirror of the plain systems
+    for (int cy = 0; cy < cycles && !rc; cy++) {
+        if (c->mode == SDMD_MODE_FAST) rc = sdmd_fast_launch_events(c, interval);
+        else {
+            DISPATCH(launch_run, c, (long long)-interval, 1LL << 62, INFINIT, INFINITY);
+            rc = hip_ok(hipGetLastError());
+        }
+        if (!rc) rc = energy_launch(c, c->dev_energy);
+        if (!rc) {
+            hipLaunchKernelGGL(k_remd_exchange, dim3(1), dim3(64), 0, c->stream, c->dev_energy, n, dT, dR, dC, dP,
+                               dH ? dH + (size_t)cy * n : nullptr, dE ? dE + (size_t)cy * n : nullptr, dErr);
+            rc = hip_ok(hipGetLastError());
+        }
+    }
+    if (!rc) rc = hip_ok(hipStreamSynchronize(c->stream));
+    int herr = 0;
+    if (!rc) rc = hip_ok(hipMemcpy(&herr, dErr, sizeof(int), hipMemcpyDeviceToHost));
+    if (!rc) rc = hip_ok(hipMemcpy(temps, dT, sizeof(sdmd_remd_temp) * n, hipMemcpyDeviceToHost));
+    if (!rc) rc = hip_ok(hipMemcpy(rng, dR, sizeof(sdmd_rand_state), hipMemcpyDeviceToHost));
+    if (!rc && counts) rc = hip_ok(hipMemcpy(counts, dC, sizeof(long long) * 2 * n, hipMemcpyDeviceToHost));
+    if (!rc && history) rc = hip_ok(hipMemcpy(history, dH, sizeof(sdmd_remd_temp) * hn, hipMemcpyDeviceToHost));
+    if (!rc && epot_history) rc = hip_ok(hipMemcpy(epot_history, dE, sizeof(double) * hn, hipMemcpyDeviceToHost));
+    if (!rc) rc = c->mode == SDMD_MODE_FAST ? sdmd_fast_sync(c) : check_errors(c);
+    if (!rc && c->mode == SDMD_MODE_FAST)
+        for (int k = 0; k < n && !rc; k++)
+            if (c->host_sys[k].error) rc = SDMD_ESTATE;
+    if (!rc && herr) rc = SDMD_ESTATE;
+    HIPV(hipFree(dT)); HIPV(hipFree(dR)); HIPV(hipFree(dC)); HIPV(hipFree(dP)); HIPV(hipFree(dErr));
+    if (dH) HIPV(hipFree(dH));
+    if (dE) HIPV(hipFree(dE));
+    free(tp);
+    free(zero);
+    return rc;
 }
 
 // ---------------------------------------------------------------- allocation-free energy

@@ -4,11 +4,17 @@ The reference runs REMD as N `sDMD -REMD` processes and one `sServer`
 exchanging (E, T) over localhost TCP (REMDExec.c:111-144, REMD.c:135-192,
 Server.c:92-211).  Here the replicas are the systems of one engine context
 (batching form, normally the fast engine): every `interval` executed events
-per replica the device computes each replica's potential energy
-(CalPotenE), the host runs the server's exchange pass (Server.c rearrange,
-sdmd_remd_rearrange) and the new temperatures are set per system on the
-device (REMD.c:97-101).  Configurations stay with their system; the
-temperatures move.
+per replica each replica's potential energy (CalPotenE) is computed, the
+server's exchange pass (Server.c rearrange) runs on it and the new
+temperatures are set per system (REMD.c:97-101).  Configurations stay with
+their system; the temperatures move.
+
+    Remd.run(cycles)   all on the device (sdmd_remd_run): event windows,
+                       energies, exchange pass and temperature updates
+                       enqueued back to back, one host synchronisation
+    Remd.cycle()       one cycle with the pass on the host
+                       (sdmd_remd_rearrange): energies copied back, new
+                       temperatures sent per replica
 
     python -m sdmd_amd.remd [--replicas 64] [--cycles 20] [--interval 10000]
 
@@ -106,6 +112,23 @@ class Remd:
         self.cycles += 1
         return pe
 
+    def run(self, cycles, history=False):
+        """`cycles` exchange cycles on the device; returns the energies each
+        pass used (cycles x n) and, with history, the {num, T} table after
+        each pass."""
+        n = self.n
+        ep = np.zeros((cycles, n))
+        hist = (RemdTemp * (cycles * n))() if history else None
+        t0 = time.perf_counter()
+        check(_lib.lib().sdmd_remd_run(self.eng._h, self.T, C.byref(self.rng), self.counts.ctypes.data, int(cycles),
+                                       self.interval, hist, ep.ctypes.data), "sdmd_remd_run")
+        self.run_s = getattr(self, "run_s", 0.0) + time.perf_counter() - t0
+        self.cycles += cycles
+        if history:
+            tab = [[(hist[c * n + k].num, hist[c * n + k].T) for k in range(n)] for c in range(cycles)]
+            return ep, tab
+        return ep
+
     def close(self):
         self.eng.close()
 
@@ -121,6 +144,7 @@ def main():
     ap.add_argument("--interval", type=int, default=10000)
     ap.add_argument("--block", type=int, default=512)
     ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--host-exchange", action="store_true", help="exchange pass on the host, one cycle per call")
     args = ap.parse_args()
     t0 = time.perf_counter()
     sysm = builder.cylinder_peptides(args.chains, args.length, seed=args.seed, radius=args.radius,
@@ -135,10 +159,13 @@ def main():
     r.prepare()
     f0 = sum(r.eng.stats(k)["frame"] for k in range(r.n))
     t1 = time.perf_counter()
-    for c in range(args.cycles):
-        r.cycle()
-        if (c + 1) % 5 == 0:
-            print("remd: cycle %d" % (c + 1), file=sys.stderr, flush=True)
+    if args.host_exchange:
+        for c in range(args.cycles):
+            r.cycle()
+            if (c + 1) % 5 == 0:
+                print("remd: cycle %d" % (c + 1), file=sys.stderr, flush=True)
+    else:
+        r.run(args.cycles)
     wall = time.perf_counter() - t1
     st = [r.eng.stats(k) for k in range(r.n)]
     events = sum(s["frame"] for s in st) - f0
@@ -150,7 +177,8 @@ def main():
                     % (r.n, args.chains, args.length, systems[0].natom, args.radius, args.box_x,
                        2 * args.radius, 2 * args.radius, args.interval),
         "events": int(events), "seconds": wall, "events_per_s": events / wall,
-        "exchange_seconds": r.exchange_s, "cycles": r.cycles,
+        "exchange": "host pass per cycle" if args.host_exchange else "device (one host synchronisation)",
+        "exchange_seconds": r.exchange_s if args.host_exchange else None, "cycles": r.cycles,
         "acceptance": [float(a) for a in acc[: r.n - 1]],
         "attempts": [int(a) for a in r.counts[0:2 * (r.n - 1):2]],
         "wall_events": int(sum(s["per_type"][5] for s in st)),

@@ -12,7 +12,8 @@
     subtype for every atom; times to 1e-12 relative where the oracle solved
     the pair itself, 1e-9 where it inherited the partner's time);
   * REMD at its stated size: 64 replicas of the 8 x 20-residue cylinder
-    system, 280-450 K, five exchange cycles of 10^4 events per replica.
+    system, 280-450 K, five exchange cycles of 10^4 events per replica, the
+    exchange passes on the device.
 """
 import numpy as np
 import pytest
@@ -85,12 +86,12 @@ def test_config5_remd_64_replicas(require_gpu, tmp_path):
     r = Remd(systems, temps, seed=5, interval=10000, block=512)
     r.prepare()
     f0 = [r.eng.stats(k)["frame"] for k in range(n)]
-    for _ in range(5):
-        pe = r.cycle()
-        assert pe.shape == (n,) and np.all(np.isfinite(pe))
-        T, nums = r.temperatures()
-        assert sorted(nums.tolist()) == list(range(1, n + 1))      # a permutation of the ladder
-        assert np.allclose(np.sort(T), np.sort(temps), rtol=0, atol=0)
+    ep, tab = r.run(5, history=True)                  # exchange passes on the device
+    assert ep.shape == (5, n) and np.all(np.isfinite(ep))
+    for c in range(5):
+        nums = sorted(num for num, _ in tab[c])
+        assert nums == list(range(1, n + 1))          # a permutation of the ladder
+        assert sorted(t for _, t in tab[c]) == sorted(temps)
     for k in range(n):
         st, fs = r.eng.stats(k), r.eng.fast_stats(k)
         assert st["error"] == 0 and fs["causal"] == 0, k

@@ -169,3 +169,36 @@ def test_fast_wall_schedule_invariance(require_gpu):
 
     s, _ = golden_system("cyl8x20")
     assert_same_schedule_result(fast_schedule_states(s, 30000), 15000)
+
+
+def test_remd_device_exchange_matches_host_pass(require_gpu):
+    """The exchange pass run on the device between event windows
+    (sdmd_remd_run: no host round trip per cycle) makes, on the energies it
+    used, the same swaps, the same attempt/acceptance counts and the same
+    rand() draws as the server's pass on the host (sdmd_remd_rearrange,
+    Server.c:145-192) replayed on those energies."""
+    from sdmd_amd._lib import RandState, RemdTemp
+    from sdmd_amd.remd import Remd, ladder, rearrange
+
+    s, _ = golden_system("cyl8x20")
+    n, cycles = 8, 6
+    temps = ladder(n, 280.0, 320.0)
+    r = Remd([s] * n, temps, seed=3, interval=3000, block=256)
+    r.prepare()
+    T0 = [(t.num, t.T) for t in r.T]
+    rng0 = RandState.from_buffer_copy(bytes(r.rng))
+    ep, tab = r.run(cycles, history=True)
+    assert ep.shape == (cycles, n) and np.all(np.isfinite(ep))
+    T = (RemdTemp * n)()
+    for k, (num, t) in enumerate(T0):
+        T[k].num, T[k].T = num, t
+    counts = np.zeros(2 * n, np.int64)
+    for c in range(cycles):
+        rearrange(ep[c], T, rng0, counts)
+        assert [(t.num, t.T) for t in T] == tab[c], c
+    assert np.array_equal(counts, r.counts)
+    assert bytes(rng0) == bytes(r.rng)                  # same draws, same stream position
+    assert counts[0::2].sum() == cycles * n and counts[1::2].sum() > 0
+    for k in range(n):
+        assert r.eng.stats(k)["error"] == 0 and r.eng.fast_stats(k)["causal"] == 0
+    r.close()
