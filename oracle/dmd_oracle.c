@@ -777,6 +777,12 @@ int oracle_create_cbt(oracle *o) {
     return 0;
 }
 
+void oracle_set_tree(oracle *o, const int32_t *node, const int32_t *leaf) {
+    memcpy(o->node, node, sizeof(int) * 2 * (size_t)o->N);
+    memcpy(o->leaf, leaf, sizeof(int) * ((size_t)o->N + 1));
+    o->count = o->N;
+}
+
 void oracle_inherited(const oracle *o, int8_t *out) {
     memcpy(out, o->inh, (size_t)o->N + 1);
 }

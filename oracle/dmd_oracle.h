@@ -73,6 +73,8 @@ long oracle_run(oracle *o, long npops, oracle_pop *trace, int64_t *hashes, long 
  * the oracle was opened with) */
 int oracle_get_state(const oracle *o, sdmd_system_desc *out);
 void oracle_get_tree(const oracle *o, int32_t *node, int32_t *leaf);
+/* replace the tree (node[0..2N-1], leaf[0..N]) with a saved one */
+void oracle_set_tree(oracle *o, const int32_t *node, const int32_t *leaf);
 uint64_t oracle_state_hash(const oracle *o);
 double oracle_currenttime(const oracle *o);
 

@@ -14,7 +14,10 @@
  *   UpdateCBT (SGThread.c:120,149)  -> per-pop trace record incl. the
  *                                      re-predicted atoms; state hashes;
  *                                      stops after SDMD_MAX_POPS pops
- *   SaveData (DataSave.c)           -> disabled (no output files)
+ *   SaveData (DataSave.c)           -> disabled (no output files) unless
+ *                                      SDMD_SAVE is set; then passed through,
+ *                                      with the pop count of every
+ *                                      trajectory dump (dumps.txt)
  * Nothing here changes the reference's arithmetic or control flow.
  */
 #define _POSIX_C_SOURCE 199309L
@@ -60,7 +63,6 @@ void __wrap_srand(unsigned int s) {
     __real_srand(s);
 }
 
-void __wrap_SaveData(enum FileType type, struct FileStr *fl) { (void)type; (void)fl; }
 
 static FILE *open_out(const char *name, const char *mode) {
     char path[1200];
@@ -68,6 +70,20 @@ static FILE *open_out(const char *name, const char *mode) {
     FILE *f = fopen(path, mode);
     if (!f) { fprintf(stderr, "ref_hooks: cannot open %s\n", path); exit(3); }
     return f;
+}
+
+/* Output files are off unless SDMD_SAVE is set; then every SaveData call
+ * goes through and the pop count at each trajectory dump is logged to
+ * dumps.txt (pops since CreateCBT, executed events, clock). */
+void __real_SaveData(enum FileType type, struct FileStr *fl);
+void __wrap_SaveData(enum FileType type, struct FileStr *fl) {
+    if (!getenv("SDMD_SAVE")) return;
+    if (type == trj && g_trace) {
+        FILE *f = open_out("dumps.txt", "a");
+        fprintf(f, "%lld %ld %a\n", g_pops, frame, currenttime);
+        fclose(f);
+    }
+    __real_SaveData(type, fl);
 }
 
 static uint64_t fnv(uint64_t h, const void *p, size_t n) {
@@ -249,7 +265,8 @@ void __wrap_UpdateCBT(int *renew) {
         clock_gettime(CLOCK_MONOTONIC, &t1);
         double el = (double)(t1.tv_sec - g_t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - g_t0.tv_nsec);
         FILE *tf = open_out("timing.txt", "w");
-        fprintf(tf, "pops %lld events %ld recal %ld seconds %.6f\n", g_pops, frame, countReCal, el);
+        fprintf(tf, "pops %lld events %ld recal %ld seconds %.6f start %.6f end %.6f\n", g_pops, frame, countReCal,
+                el, (double)g_t0.tv_sec + 1e-9 * (double)g_t0.tv_nsec, (double)t1.tv_sec + 1e-9 * (double)t1.tv_nsec);
         fclose(tf);
         if (g_trace) {
             fclose(g_trace);

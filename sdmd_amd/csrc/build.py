@@ -18,7 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(PKG, "libsdmd.so")
 ARCH = os.environ.get("SDMD_OFFLOAD_ARCH", "gfx950")
 
-HOST_SOURCES = ["host/sdmd_host.c", "host/sdmd_remd.c"]
+HOST_SOURCES = ["host/sdmd_host.c", "host/sdmd_remd.c", "host/sdmd_io.c"]
 HIP_SOURCES = ["sdmd_gpu.hip", "sdmd_fast.hip"]
 
 

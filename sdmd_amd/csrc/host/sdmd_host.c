@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include "sdmd_host.h"
+#include "sdmd_host_internal.h"
 
 /* ------------------------------------------------------------------ */
 /* glibc TYPE_3 additive-feedback generator (the rand() the reference   */
@@ -77,61 +78,6 @@ static double mb_normal(sdmd_rand_state *st) {
 /* ------------------------------------------------------------------ */
 /* setup-time data model                                              */
 /* ------------------------------------------------------------------ */
-typedef struct { double d, e, acc; } step_t;
-typedef struct { int n, cap; step_t *v; } steplist_t;
-typedef struct { int j; double dmin, dmax; int steps; } link_t;   /* steps: index in pool, -1 none */
-typedef struct { int n, cap; link_t *v; } linklist_t;              /* index 0 = list head */
-typedef struct { int j, bits; } cm_t;
-typedef struct { int n, cap; cm_t *v; } cmrow_t;
-
-typedef struct {
-    char name[8], aa[8], extra0[16], extra1[16];
-    int type, aatype, protein, aanum;
-    double mass;
-} atomprop_t;
-
-typedef struct { char name[8]; int type, start, end, protein; } aa_t;
-typedef struct { int startAA, endAA, startAtom, endAtom; } pep_t;
-typedef struct { double dmin, dmax; int steps; int set; } tab_t;
-
-struct sdmd_host_system {
-    char datadir[1024];
-    /* parameters */
-    char neworcontinue[20], method[20], thermostat[20], temptype[20];
-    char wallexist[20], walltype[20];
-    int wall_type;                 /* ReadWall: AtomModel of Wall.txt's type name */
-    double wall_mass, wall_pos[4];
-    double timestep, outputrate, cutoffr, thermoF, T;
-    /* atoms */
-    int natom, naa, nprot;
-    atomprop_t *p;
-    aa_t *aa;
-    pep_t *pep;
-    double *x, *y, *z, *vx, *vy, *vz;
-    int *cx, *cy, *cz;
-    char *role;
-    int *hbnb;
-    linklist_t *bond, *constr;
-    cmrow_t *cmap;
-    /* tables */
-    int nsteplist, capsteplist;
-    steplist_t *steplists;
-    tab_t pair[SDMD_NTYPE][SDMD_NTYPE];
-    tab_t hbpair[SDMD_NHBCLASS][SDMD_NTYPE][SDMD_NTYPE];
-    double hbpot[6];
-    double box[4];
-    int cellnum[4];
-    double cellsize[4];
-    int *celllist;
-    sdmd_rand_state rng;
-    uint32_t seed;
-    /* flattened view */
-    sdmd_system_desc desc;
-    void *flat_mem[64];
-    int nflat;
-    char *err;
-    int errlen;
-};
 
 static int fail(sdmd_host_system *s, int code, const char *fmt, ...) {
     if (s && s->err && s->errlen > 0) {
@@ -327,8 +273,8 @@ static int read_parameter(sdmd_host_system *s) {
     s->thermoF = atof(fields[7]);
     snprintf(s->wallexist, sizeof s->wallexist, "%s", fields[9]);
     snprintf(s->walltype, sizeof s->walltype, "%s", fields[10]);
-    if (strcmp(s->neworcontinue, "new"))
-        return fail(s, SDMD_EUNSUPPORTED, "only 'new' runs are supported (got %s)", s->neworcontinue);
+    if (strncmp(s->neworcontinue, "new", 1) && strncmp(s->neworcontinue, "continue", 1))
+        return fail(s, SDMD_EFORMAT, "Continue? must be 'new' or 'continue' (got %s)", s->neworcontinue);
     if (strncmp(s->method, "Ding", 1))
         return fail(s, SDMD_EUNSUPPORTED, "only the Ding force field is supported");
     if (strcmp(fields[8], "no"))
@@ -631,7 +577,7 @@ static int scan_aa(sdmd_host_system *s, int AANum, int type) {
     const char *nextAA = aa_name(s, AANum + 1);
     while (rd_tok(&r, tok, sizeof tok)) {
         if (strcmp(tok, "ATOM") == 0) {
-            if (type == 0) {
+            if (type == 0 && !s->cont) {    /* a continue run keeps the saved types and masses */
                 char tname[64], e0[64], e1[64];
                 double mass = 0;
                 rd_tok(&r, n1, sizeof n1);
@@ -914,26 +860,83 @@ static int read_wall(sdmd_host_system *s) {
     return SDMD_OK;
 }
 
-int sdmd_host_load(const char *datadir, const char *gro_name, uint32_t seed,
-                   sdmd_host_system **out, char *err, int errlen) {
-    if (!datadir || !gro_name || !out) return SDMD_EINVAL;
+int sdmd_host_fail(sdmd_host_system *s, int code, const char *fmt, ...) {
+    if (s && s->err && s->errlen > 0) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(s->err, (size_t)s->errlen, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+int *sdmd_host_cm_ref(cmrow_t *row, int j) { return cm_ref(row, j); }
+
+int sdmd_host_begin(const char *datadir, sdmd_host_system **out, char *err, int errlen) {
+    if (!datadir || !out) return SDMD_EINVAL;
     sdmd_host_system *s = calloc(1, sizeof *s);
     if (!s) return SDMD_ENOMEM;
     s->err = err;
     s->errlen = errlen;
     if (err && errlen > 0) err[0] = 0;
     snprintf(s->datadir, sizeof s->datadir, "%s", datadir);
+    int rc = read_parameter(s);
+    if (rc) { sdmd_host_free(s); return rc; }
+    *out = s;
+    return SDMD_OK;
+}
+
+/* ReadModel, CreatePotentialMatrix, ReadHB and ReadWall in the order both
+ * branches of InputData call them (Initialization.c:292-303, 410-423) */
+int sdmd_host_finish_model(sdmd_host_system *s) {
     int rc;
-    if ((rc = read_parameter(s))) goto bad;
+    if ((rc = read_model(s))) return rc;
+    if ((rc = create_potential_matrix(s))) return rc;
+    if ((rc = read_hb(s))) return rc;
+    if (strcmp(s->wallexist, "no") && (rc = read_wall(s))) return rc;
+    return SDMD_OK;
+}
+
+static double mass_of(const sdmd_host_system *s, int lo, int hi) {
+    double m = 0;
+    for (int n = lo; n <= hi; n++) {
+        if (strncmp(s->p[n].name, "H", 1) == 0) m++;
+        else m += s->p[n].mass;
+    }
+    return m;
+}
+
+double sdmd_host_residue_mass(const sdmd_host_system *s, int aa) { return mass_of(s, s->aa[aa].start, s->aa[aa].end); }
+double sdmd_host_chain_mass(const sdmd_host_system *s, int c) { return mass_of(s, s->pep[c].startAtom, s->pep[c].endAtom); }
+
+double sdmd_host_charge(const sdmd_host_system *s, int i) {
+    /* chargeAA[] all 0: ChargeAACheck is 0, so the template sign applies,
+       and the terminal rule (!chargeAA[TER]) marks each chain's first N
+       (type 18) +1 and its last O-type 26 atom -1 */
+    const int c = s->p[i].protein;
+    if (c >= 1 && c <= s->nprot) {
+        for (int n = s->pep[c].startAtom; n <= s->pep[c].endAtom; n++)
+            if (s->p[n].type == 18) { if (n == i) return 1; break; }
+        for (int n = s->pep[c].endAtom; n >= s->pep[c].startAtom; n--)
+            if (s->p[n].type == 26) { if (n == i) return -1; break; }
+    }
+    if (!strcmp(s->p[i].extra0, "+")) return 1;
+    if (!strcmp(s->p[i].extra0, "-")) return -1;
+    return 0;
+}
+
+int sdmd_host_load(const char *datadir, const char *gro_name, uint32_t seed,
+                   sdmd_host_system **out, char *err, int errlen) {
+    if (!datadir || !gro_name || !out) return SDMD_EINVAL;
+    sdmd_host_system *s = NULL;
+    int rc = sdmd_host_begin(datadir, &s, err, errlen);
+    if (rc) return rc;
     s->seed = seed;
     sdmd_rand_seed(&s->rng, seed);
     if ((rc = read_gro(s, gro_name))) goto bad;
     if (s->natom < 2) { rc = fail(s, SDMD_EFORMAT, "no atoms"); goto bad; }
     create_cell(s);
-    if ((rc = read_model(s))) goto bad;
-    if ((rc = create_potential_matrix(s))) goto bad;
-    if ((rc = read_hb(s))) goto bad;
-    if (strcmp(s->wallexist, "no") && (rc = read_wall(s))) goto bad;
+    if ((rc = sdmd_host_finish_model(s))) goto bad;
     for (int i = 1; i <= s->natom; i++) hb_neighbor_assign(s, i);
     generate_velocity(s);
     *out = s;
@@ -1345,6 +1348,24 @@ const sdmd_system_desc *sdmd_host_desc(sdmd_host_system *s) {
         d->hb_neighbor[i] = s->hbnb[i];
         d->ev_partner[i] = SDMD_INVALID;   /* ReadCoordinate 1019-1022 */
         d->ev_time[i] = SDMD_INFINIT;
+        if (s->cont) {                     /* a continue run: the saved dynamic state */
+            d->hb_bond[i] = s->hbbond[i];
+            d->hb_itype[i] = s->hbitype[i];
+            for (int k = 0; k < SDMD_NBSLOTS; k++) d->nb_partner[i * SDMD_NBSLOTS + k] = s->nbp[i * SDMD_NBSLOTS + k];
+            d->nb_partnernum[i] = s->nbnum[i];
+            d->nb_status[i] = s->nbstat[i];
+            for (int k = 0; k < SDMD_DYNSLOTS; k++) {
+                d->dyn_j[i * SDMD_DYNSLOTS + k] = s->dynj[i * SDMD_DYNSLOTS + k];
+                d->dyn_bits[i * SDMD_DYNSLOTS + k] = s->dynb[i * SDMD_DYNSLOTS + k];
+            }
+            d->ev_partner[i] = s->evp[i];
+            d->ev_counter[i] = s->evc[i];
+            d->ev_pcounter[i] = s->evpc[i];
+            d->ev_time[i] = s->evt[i];
+            d->ev_pot[i] = s->evpot[i];
+            d->ev_type[i] = s->evtype[i];
+            d->ev_sub[i] = s->evsub[i];
+        }
     }
     long ncell = (long)s->cellnum[1] * s->cellnum[2] * s->cellnum[3];
     d->ncelllist = (int32_t)(N + ncell + 1);
@@ -1415,7 +1436,7 @@ const sdmd_system_desc *sdmd_host_desc(sdmd_host_system *s) {
     free(base);
 #undef ARR
 #undef TAB
-    d->currenttime = 0;
+    d->currenttime = s->cont ? s->currenttime : 0;
     d->seed = s->seed;
     d->rand_calls = s->rng.calls;
     return d;
@@ -1434,6 +1455,9 @@ void sdmd_host_free(sdmd_host_system *s) {
     free(s->x); free(s->y); free(s->z); free(s->vx); free(s->vy); free(s->vz);
     free(s->cx); free(s->cy); free(s->cz); free(s->role); free(s->hbnb);
     free(s->celllist);
+    void *cont[] = {s->hbbond, s->hbitype, s->nbp, s->nbnum, s->nbstat, s->dynj, s->dynb, s->evp, s->evtype,
+                    s->evsub, s->evc, s->evpc, s->evt, s->evpot};
+    for (size_t k = 0; k < sizeof cont / sizeof cont[0]; k++) free(cont[k]);
     free(s);
 }
 

@@ -159,6 +159,28 @@ class GpuEngine:
         """Run until the clock passes t_end (at most max_pops pops per system)."""
         check(_lib.lib().sdmd_gpu_run_until(self._h, float(t_end), int(max_pops)), "sdmd_gpu_run_until")
 
+    def run_to_output(self, t_out, t_end=1e300, max_pops=1 << 40):
+        """Plain mode: pop while the clock is at most t_end and stop after
+        the first executed event that leaves it at or past t_out (where
+        SingleThreadRun writes its output files, SGThread.c:130-139)."""
+        check(_lib.lib().sdmd_gpu_run_to_output(self._h, float(t_out), float(t_end), int(max_pops)),
+              "sdmd_gpu_run_to_output")
+
+    def set_tree(self, sys, node, leaf):
+        """Plain mode: replace a system's event tree (a resumed run's)."""
+        node = np.ascontiguousarray(node, dtype=np.int32)
+        leaf = np.ascontiguousarray(leaf, dtype=np.int32)
+        check(_lib.lib().sdmd_gpu_set_tree(self._h, sys, node.ctypes.data, leaf.ctypes.data), "sdmd_gpu_set_tree")
+
+    def rand_position(self, sys=0):
+        """Plain mode: rand() calls since srand(seed) consumed so far by the
+        thermostat draws (the stream position a resumed run starts from)."""
+        used = self.stats(sys)["normals_used"]
+        calls = self._calls[sys]
+        if used == 0 or calls is None:
+            return int(self._rand0[sys])
+        return int(calls[used - 1])
+
     def run_events(self, events, max_ms=0.0):
         """Run until every system executed `events` more events (or max_ms of
         device time passed)."""
@@ -253,7 +275,7 @@ class GpuEngine:
         return {"pops": st.pops, "frame": st.frame, "recal": st.recal,
                 "per_type": np.array(st.per_type[:]), "pair_evals": st.pair_evals,
                 "candidates": st.candidates, "normals_used": st.normals_used,
-                "currenttime": st.currenttime, "error": st.error}
+                "currenttime": st.currenttime, "error": st.error, "last_exec": bool(st.last_exec)}
 
     def trace(self, sys=0):
         n = C.c_int64()

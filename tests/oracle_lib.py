@@ -50,6 +50,7 @@ def lib():
         L.oracle_run.argtypes = [P, C.c_long, C.c_void_p, C.c_void_p, C.c_long]
         L.oracle_get_state.argtypes = [P, C.POINTER(SystemDesc)]
         L.oracle_get_tree.argtypes = [P, C.c_void_p, C.c_void_p]
+        L.oracle_set_tree.argtypes = [P, C.c_void_p, C.c_void_p]
         L.oracle_state_hash.restype = C.c_uint64
         L.oracle_state_hash.argtypes = [P]
         L.oracle_currenttime.restype = C.c_double
@@ -126,6 +127,11 @@ class Oracle:
         leaf = np.zeros(N + 1, np.int32)
         lib().oracle_get_tree(self.h, node.ctypes.data, leaf.ctypes.data)
         return node, leaf
+
+    def set_tree(self, node, leaf):
+        node = np.ascontiguousarray(node, dtype=np.int32)
+        leaf = np.ascontiguousarray(leaf, dtype=np.int32)
+        lib().oracle_set_tree(self.h, node.ctypes.data, leaf.ctypes.data)
 
     def state_hash(self):
         return lib().oracle_state_hash(self.h)

@@ -14,7 +14,7 @@ from conftest import ROOT
 
 def declared_functions():
     names = set()
-    for h in ("sdmd_host.h", "sdmd_gpu.h", "sdmd_remd.h"):
+    for h in ("sdmd_host.h", "sdmd_gpu.h", "sdmd_remd.h", "sdmd_io.h"):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"\b(sdmd_\w+)\s*\(", text):
