@@ -16,17 +16,22 @@ time of the timed steps.  The plain FP64 engine (bit-exact with the
 reference) runs the same systems for the same windows; `plain` and
 `vs_plain` report it.
 
-`roofline` prices the dominant kernel (the fast event loop) with algorithmic
-bytes per launch: every re-predicted atom reads its own state (96 B), the
-32-B FP32 mirror record of each neighbour-cell candidate, 56 B of FP64 state
-per FP64 re-solve, writes its 40-B event record and rescans one 64-atom block
-of event times (512 B); every executed event reads and writes the state of
-its atoms (2 x 104 B).  `cpu_baseline` times the reference built from its own
-sources (oracle/_ref/sdmd_ref) on one core, on a bounded sample (20000 pops)
-of the median-size system of the same batch.
+`roofline` prices the dominant kernel (the fast event loop) with the
+algorithmic bytes of the hot-path scope: every re-predicted atom reads its
+candidates' state, (n_nb + 1) x 56 B (position, velocity, type and
+exclusion word of each of the n_nb candidates scanned and of the atom
+itself); every executed event writes a 40-B event record and walks a tree
+path of 2 ceil(log2 N) x 12 B.  The counts behind it (events, predictions,
+candidates, confirms, one-body events, stale pops, launches) are printed, so
+the figure can be recomputed from the line.  `cpu_baseline` runs the
+reference built from its own sources (oracle/_ref/sdmd_ref) as P = min(#systems,
+host cores) concurrent single-threaded processes pinned to one core each,
+one system of the batch per process (spread over the size range), each for a
+bounded number of pops, and reports the sum of their event rates.
 """
 import argparse
 import json
+import math
 import os
 import subprocess
 import sys
@@ -68,50 +73,86 @@ def load_systems(dirs):
     return systems, time.time() - t0
 
 
-def cpu_baseline(indir, pops):
-    """The reference (own sources, -O2) on one core, or the CPU oracle."""
-    ref = os.path.join(ROOT, "oracle", "_ref", "sdmd_ref")
-    out = tempfile.mkdtemp(prefix="sdmd_cpu_")
-    if os.path.exists(ref):
-        env = dict(os.environ, SDMD_LIGHT="1", SDMD_DUMP=out, SDMD_MAX_POPS=str(pops))
-        cmd = [ref, "-f", "coordinate.gro", "-i", indir, "-o", out]
-        if subprocess.run(["which", "taskset"], capture_output=True).returncode == 0:
-            cmd = ["taskset", "-c", "0"] + cmd
-        with open(os.path.join(out, "stdout.txt"), "w") as fh:
-            subprocess.run(cmd, env=env, stdout=fh, stderr=subprocess.STDOUT, check=True)
-        tok = open(os.path.join(out, "timing.txt")).read().split()
-        vals = dict(zip(tok[0::2], tok[1::2]))
-        ev, sec = int(vals["events"]), float(vals["seconds"])
-        return {"value": ev / sec, "unit": "events/s", "cores": 1, "kind": "reference",
-                "sample": "reference sDMD (own sources, -O2) on %s, first %d pops (%d events) of its event "
-                          "loop, %.2f s, one core" % (os.path.basename(indir), pops, ev, sec)}
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib
-    from sdmd_amd.engine import System
+def host_cpus():
+    """(cores this process may use, CPU model)."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return cores, model
 
-    s = System.load(indir)
-    o = oracle_lib.Oracle(s.arrays)
-    o.bulk()
-    t0 = time.perf_counter()
-    o.run(pops, trace=False)
-    sec = time.perf_counter() - t0
-    ev = int(o.counters()["per_type"].sum())
-    return {"value": ev / sec, "unit": "events/s", "cores": 1, "kind": "port",
-            "sample": "CPU oracle on %s, first %d pops (%d events), %.2f s, one core" % (
-                os.path.basename(indir), pops, ev, sec)}
+
+def cpu_baseline(dirs, natoms, seconds, procs):
+    """The reference (own sources, -O2) as `procs` concurrent processes, one
+    system each, pinned to one core each, each for about `seconds` of its
+    event loop; the sum of their loop rates."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "sdmd_ref")
+    if not os.path.exists(ref):
+        raise SystemExit("bench.py: oracle/_ref/sdmd_ref (the reference built from its own sources by "
+                         "`make -C oracle ref`) is missing; the CPU baseline needs it")
+    cores, model = host_cpus()
+    try:
+        allowed = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = list(range(cores))
+    P = max(1, min(procs, len(dirs), len(allowed)))
+    order = sorted(range(len(dirs)), key=lambda k: natoms[k])
+    pick = [order[int(round(q * (len(order) - 1) / max(P - 1, 1)))] for q in range(P)]
+    have_taskset = subprocess.run(["which", "taskset"], capture_output=True).returncode == 0
+    runs = []
+    for q, k in enumerate(pick):
+        out = tempfile.mkdtemp(prefix="sdmd_cpu_")
+        # a pop costs about 12.5 ns per atom (the O(N) position sweep) + 5 us on one core
+        pops = int(seconds / (12.5e-9 * natoms[k] + 5e-6))
+        env = dict(os.environ, SDMD_LIGHT="1", SDMD_DUMP=out, SDMD_MAX_POPS=str(pops))
+        cmd = [ref, "-f", "coordinate.gro", "-i", dirs[k], "-o", out]
+        if have_taskset:
+            cmd = ["taskset", "-c", str(allowed[q])] + cmd
+        fh = open(os.path.join(out, "stdout.txt"), "w")
+        runs.append((k, out, subprocess.Popen(cmd, env=env, stdout=fh, stderr=subprocess.STDOUT), fh))
+    rates, events, t_lo, t_hi, npops = [], 0, [], [], 0
+    for k, out, p, fh in runs:
+        if p.wait() != 0:
+            raise SystemExit("bench.py: the reference failed on %s (see %s)" % (dirs[k], out))
+        fh.close()
+        tok = open(os.path.join(out, "timing.txt")).read().split()
+        v = dict(zip(tok[0::2], tok[1::2]))
+        rates.append(int(v["events"]) / float(v["seconds"]))
+        events += int(v["events"])
+        npops += int(v["pops"])
+        t_lo.append(float(v["start"]))
+        t_hi.append(float(v["end"]))
+    overlap = max(0.0, min(t_hi) - max(t_lo))
+    return {"value": sum(rates), "unit": "events/s", "cores": P, "kind": "reference",
+            "host_cores": cores, "cpu_model": model,
+            "sample": "reference sDMD (own sources, -O2), %d concurrent single-threaded processes pinned to one "
+                      "core each, one batch system each (%d-%d atoms), about %.0f s of each event loop (%d pops, "
+                      "%d events in all); value = sum of the %d loop rates (all loops ran together for %.1f s)"
+                      % (P, min(natoms[k] for k in pick), max(natoms[k] for k in pick), seconds, npops, events, P,
+                         overlap),
+            "per_process_events_per_s": [round(r, 1) for r in rates]}
 
 
 def totals(eng, n, fast):
     t = {"frame": 0, "pops": 0, "recal": 0, "pair_evals": 0, "candidates": 0, "onebody": 0,
-         "screens": 0, "confirms": 0, "batches": 0, "causal": 0, "errors": set()}
+         "screens": 0, "confirms": 0, "batches": 0, "causal": 0, "errors": set(), "frame_per_sys": [0] * n}
     for k in range(n):
         st = eng.stats(k)
         t["frame"] += st["frame"]
+        t["frame_per_sys"][k] = st["frame"]
         t["pops"] += st["pops"]
         t["recal"] += st["recal"]
         t["pair_evals"] += st["pair_evals"]
         t["candidates"] += st["candidates"]
-        t["onebody"] += int(st["per_type"][4] + st["per_type"][10])     # thermostat, cell crossing
+        t["onebody"] += int(st["per_type"][4] + st["per_type"][5] + st["per_type"][10])   # thermostat, wall, cell crossing
         t["errors"].add(st["error"])
         if fast:
             fs = eng.fast_stats(k)
@@ -121,7 +162,9 @@ def totals(eng, n, fast):
 
 
 def diff(a, b):
-    return {k: (a[k] - b[k]) for k in a if k != "errors"}
+    out = {k: (a[k] - b[k]) for k in a if k not in ("errors", "frame_per_sys")}
+    out["frame_per_sys"] = [x - y for x, y in zip(a["frame_per_sys"], b["frame_per_sys"])]
+    return out
 
 
 def timed(eng, nsys, fast, steps, warmup, step_ms, graph=0):
@@ -172,7 +215,9 @@ def main():
                                                          "(0: one launch per step)")
     ap.add_argument("--plain-block", type=int, default=512, help="plain engine threads per system (512 measured fastest of 256/512/1024)")
     ap.add_argument("--systems", type=int, default=48)
-    ap.add_argument("--cpu-pops", type=int, default=600000)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="event-loop time per reference process")
+    ap.add_argument("--cpu-procs", type=int, default=48, help="most concurrent reference processes "
+                                                              "(capped by the batch and the usable cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -209,12 +254,16 @@ def main():
         args.block, depth = eng.autotune(args.autotune_ms)
     f, elapsed, kernel_ms = timed(eng, nsys, True, args.steps, args.warmup, args.step_ms, graph=args.graph)
     assert f["errors"] == {0}, f["errors"]
+    assert f["causal"] == 0, f["causal"]             # no read of an atom later than the reader
+    helpers = eng.fast_stats(0)["helpers"]
     eng.close()
     events = f["frame"]
     predictions = 2 * events + f["recal"] - f["onebody"]
     n_nb = f["candidates"] / max(predictions, 1)
     launches = args.steps * max(args.graph, 1)
-    bytes_launch = (predictions * (96 + n_nb * 32 + 40 + 512) + f["confirms"] * 56 + events * 2 * 104) / launches
+    # algorithmic bytes: (n_nb + 1) x 56 B per prediction, 40 B + 2 ceil(log2 N) x 12 B per executed event
+    tree_bytes = sum(f["frame_per_sys"][k] * 2 * math.ceil(math.log2(natoms[k])) * 12 for k in range(nsys))
+    bytes_launch = (f["candidates"] * 56 + predictions * 56 + events * 40 + tree_bytes) / launches
     kernel_ms_launch = kernel_ms / launches
     achieved = bytes_launch / (kernel_ms_launch / 1e3) / 1e9
     traffic = None
@@ -255,7 +304,13 @@ def main():
         "setup_s": {"build": tbuild, "host": thost},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "bytes_per_launch": bytes_launch, "kernel_ms_per_launch": kernel_ms_launch},
+                     "bytes_per_launch": bytes_launch, "kernel_ms_per_launch": kernel_ms_launch,
+                     "formula": "((candidates + predictions) * 56 + events * 40 + sum_s events_s * 24 * "
+                                "ceil(log2 N_s)) / launches",
+                     "counts": {"events": events, "predictions": predictions, "candidates": f["candidates"],
+                                "confirms": f["confirms"], "onebody": f["onebody"], "stale": f["recal"],
+                                "launches": launches, "tree_bytes": tree_bytes}},
+        "helpers_per_system": helpers,
     }
     if not args.no_plain:
         pops_cap = int(40000 * (args.steps + args.warmup) * args.step_ms / 1000.0) + 1000
@@ -268,8 +323,7 @@ def main():
                          "pops_per_s": p["pops"] / pel, "block": args.plain_block}
         line["vs_plain"] = line["value"] / line["plain"]["value"]
     if not args.no_cpu_baseline:
-        order = sorted(range(nsys), key=lambda k: natoms[k])
-        line["cpu_baseline"] = cpu_baseline(dirs[order[nsys // 2]], args.cpu_pops)
+        line["cpu_baseline"] = cpu_baseline(dirs, natoms, args.cpu_seconds, args.cpu_procs)
     print(json.dumps(line))
 
 
