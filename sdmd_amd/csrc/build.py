@@ -52,6 +52,12 @@ def build(verbose=True):
     tmp = LIB + ".tmp"
     _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lm"])
     os.replace(tmp, LIB)
+    # the C caller of the fine-grained entry points (tests/c_abi_loop.c),
+    # linked against the library with only include/*.h
+    drv = os.path.join(ROOT, "tests", "c_abi_loop.c")
+    if os.path.exists(drv):
+        _run(["gcc", "-O2", "-std=c99", "-Wall", "-I" + INC, drv, "-o", os.path.join(ROOT, "tests", "c_abi_loop.bin"),
+              "-L" + PKG, "-lsdmd", "-Wl,-rpath," + PKG])
     return LIB
 
 
