@@ -76,7 +76,8 @@ typedef struct {
     uint64_t seed;         /* fast: thermostat stream key (0 = the system's seed) */
     int32_t helpers;       /* fast: helper workgroups per system that take re-predictions on other
                               CUs (0 = as many as the GPU holds beside the owners, -1 = none) */
-    int32_t pad;
+    int32_t window;        /* plain: neighbour candidates scanned per pass of a prediction (0 = 1728,
+                              the staging area; 1..1728: more passes, the same result) */
 } sdmd_gpu_opts;
 
 /* one popped event (same layout as the reference trace dump) */

@@ -39,7 +39,7 @@ class GpuOpts(C.Structure):
     _fields_ = [("mode", C.c_int32), ("block", C.c_int32), ("trace", C.c_int32),
                 ("profile", C.c_int32), ("trace_capacity", C.c_int64), ("hash_every", C.c_int64),
                 ("validate", C.c_int32), ("max_batch", C.c_int32), ("seed", C.c_uint64),
-                ("helpers", C.c_int32), ("pad", C.c_int32)]
+                ("helpers", C.c_int32), ("window", C.c_int32)]
 
 
 class DatInfo(C.Structure):

@@ -553,6 +553,19 @@ def ala10_extended(seed=1, box=50.0):
     return System([ch], (box, box, box), "ALA10 extended chain")
 
 
+def single_chain(seq, seed, box):
+    """One chain of the given sequence (fragment assembly with seeded
+    backbone torsions) centred in a cubic box of edge `box` (A).  Bulky
+    residues with a large cut-off give dense cells: TRP-ARG-LYS-PHE-TYR x 6
+    (431 atoms, seed 8) in a 72 A box with a 12 A cut-off puts 76 atoms in
+    its fullest cell."""
+    ch = build_chains([list(seq)], seed, workers=1)[0]
+    if 2 * np.max(np.linalg.norm(ch.xyz, axis=1)) >= box:
+        raise RuntimeError("chain too large for the box")
+    ch.xyz = ch.xyz + np.array([box / 2, box / 2, box / 2])
+    return System([ch], (box, box, box), "single chain seed %d" % seed)
+
+
 def ala10_cylinder(seed=1, box=(50.0, 24.0, 24.0)):
     """ALA10 extended chain inside a smooth cylinder wall of radius box_y/2
     along x: the chain's long axis is turned onto x and centred on the

@@ -51,6 +51,7 @@ struct Sys {
     double cellsize[4], box[4];
     double cutoffr, T, thermoF;
     int thermostat;
+    int win;                    // candidates per pass of the windowed neighbour scan (<= MAXC)
     double *x, *y, *z, *vx, *vy, *vz, *mass;
     int *type, *aatype, *protein, *aanum, *cx, *cy, *cz;
     signed char *role;

@@ -48,7 +48,8 @@ namespace sdmd {
 
 typedef float fvec4 __attribute__((ext_vector_type(4)));   // mirror record half (16 B)
 
-constexpr int FCAP = 64;          // atoms per cell
+constexpr int FCAP_MIN = 64;      // slots per cell at least (a system's capacity: FSys.cap)
+constexpr int FCAP_MAX = 1024;    // ... at most
 constexpr int FMAXBLK = 1024;     // 64-atom blocks per system (N <= 65535)
 constexpr int FSEL = 64;          // events considered per round
 constexpr int FROW = 96;          // connection-map row staged per wave
@@ -80,7 +81,8 @@ struct __attribute__((aligned(16))) ConRad {
 struct FSys {
     Sys s;
     double *t0;                 // local time of (x, v)
-    int *ccnt, *cmem, *slot;    // cell arrays: ccnt[cell], cmem[cell*FCAP+q], slot[atom]
+    int *ccnt, *cmem, *slot;    // cell arrays: ccnt[cell], cmem[cell*cap+q], slot[atom]
+    int cap;                    // slots per cell: a power of two, twice the start's fullest cell or more
     fvec4 *rec;                // FP32 mirror, two fvec4 per slot
     double *b1, *b2, *b3;       // block's earliest, second and third event times (persisted)
     int *ba, *bb;               // atoms of the earliest and second event
@@ -174,10 +176,10 @@ struct PosLocal {      // positions at time t from the local-time state, cells f
         p[1] = G(S.y)[j] + G(S.vy)[j] * dt;
         p[2] = G(S.z)[j] + G(S.vz)[j] * dt;
     }
-    __device__ int first(int cell) const { return G(F.ccnt)[cell] > 0 ? G(F.cmem)[(size_t)cell * 64] : 0; }
+    __device__ int first(int cell) const { return G(F.ccnt)[cell] > 0 ? G(F.cmem)[(size_t)cell * F.cap] : 0; }
     __device__ int next(int cell, int j) const {
         const int q = G(F.slot)[j] + 1;
-        return (q > 0 && q < G(F.ccnt)[cell] && q < 64) ? G(F.cmem)[(size_t)cell * 64 + q] : 0;
+        return (q > 0 && q < G(F.ccnt)[cell] && q < F.cap) ? G(F.cmem)[(size_t)cell * F.cap + q] : 0;
     }
 };
 
@@ -194,7 +196,7 @@ __device__ __forceinline__ void advance(FSys &F, int a, double t) {
 __device__ void put_record(FSys &F, int a) {
     const Sys &S = F.s;
     const int cx = G(S.cx)[a], cy = G(S.cy)[a], cz = G(S.cz)[a];
-    const size_t at = (size_t)cell_id(S, cx, cy, cz) * FCAP + G(F.slot)[a];
+    const size_t at = (size_t)cell_id(S, cx, cy, cz) * F.cap + G(F.slot)[a];
     fvec4 r0, r1;
     r0.x = (float)(G(S.x)[a] - cx * S.cellsize[1]);
     r0.y = (float)(G(S.y)[a] - cy * S.cellsize[2]);
@@ -888,7 +890,7 @@ __device__ __forceinline__ void collision_fast(FSys &F, const ScrTab &T, FWave &
         for (int q = 0; q < FQ; q++) {
             const int k = min(lane + 64 * q, n - 1);
             const int c = cand_cell(wm, k);
-            const size_t at = (size_t)wm.cell[c] * FCAP + (k - wm.coff[c]);
+            const size_t at = (size_t)wm.cell[c] * F.cap + (k - wm.coff[c]);
             cq[q] = c;
             r0[q] = G(F.rec)[2 * at];
             r1[q] = G(F.rec)[2 * at + 1];
@@ -930,7 +932,7 @@ __device__ __forceinline__ void collision_fast(FSys &F, const ScrTab &T, FWave &
         float thi = FINF;
         for (int k = lane; k < n; k += 64) {
             const int c = cand_cell(wm, k);
-            const size_t at = (size_t)wm.cell[c] * FCAP + (k - wm.coff[c]);
+            const size_t at = (size_t)wm.cell[c] * F.cap + (k - wm.coff[c]);
             CandS o;
             const fvec4 q0 = G(F.rec)[2 * at], q1 = G(F.rec)[2 * at + 1];
             screen_at(F, T, wm, A, c, q0, q1, s0, s1, tA, calib, cband, o, my_causal);
@@ -941,7 +943,7 @@ __device__ __forceinline__ void collision_fast(FSys &F, const ScrTab &T, FWave &
         int dummy = 0;
         for (int k = lane; k < n; k += 64) {
             const int c = cand_cell(wm, k);
-            const size_t at = (size_t)wm.cell[c] * FCAP + (k - wm.coff[c]);
+            const size_t at = (size_t)wm.cell[c] * F.cap + (k - wm.coff[c]);
             CandS o;
             const fvec4 q0 = G(F.rec)[2 * at], q1 = G(F.rec)[2 * at + 1];
             screen_at(F, T, wm, A, c, q0, q1, s0, s1, tA, calib, cband, o, dummy);
@@ -1160,7 +1162,7 @@ __device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm
             int my = 0;
             for (int k = lane; k < n; k += 64) {
                 const int c = cand_cell(wm, k);
-                const size_t at = (size_t)wm.cell[c] * FCAP + (k - wm.coff[c]);
+                const size_t at = (size_t)wm.cell[c] * F.cap + (k - wm.coff[c]);
                 const int j = __float_as_int(G(F.rec)[2 * at + 1].w) >> 8;
                 if (j < 1 || j > S.N) continue;
                 double pot;
@@ -1236,8 +1238,8 @@ __device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm
 __device__ void cell_remove(FSys &F, int a, int cell, int *err) {
     const int q = G(F.slot)[a];
     const int last = G(F.ccnt)[cell] - 1;
-    if (q < 0 || q > last || last >= FCAP) { *err = FERR_SLOT; return; }
-    const size_t base = (size_t)cell * FCAP;
+    if (q < 0 || q > last || last >= F.cap) { *err = FERR_SLOT; return; }
+    const size_t base = (size_t)cell * F.cap;
     if (q != last) {
         const int m = G(F.cmem)[base + last];
         G(F.cmem)[base + q] = m;
@@ -1250,8 +1252,8 @@ __device__ void cell_remove(FSys &F, int a, int cell, int *err) {
 
 __device__ void cell_insert(FSys &F, int a, int cell, int *err) {
     const int q = G(F.ccnt)[cell];
-    if (q >= FCAP) { *err = ERR_CELLCAP; return; }
-    G(F.cmem)[(size_t)cell * FCAP + q] = a;
+    if (q >= F.cap) { *err = ERR_CELLCAP; return; }
+    G(F.cmem)[(size_t)cell * F.cap + q] = a;
     G(F.slot)[a] = q;
     G(F.ccnt)[cell] = q + 1;
 }
@@ -1532,7 +1534,7 @@ __device__ void rebase(FSys &F, double Te) {
     const Sys &S = F.s;
     for (int a = threadIdx.x + 1; a <= S.N; a += 64 * W) {
         const int cx = G(S.cx)[a], cy = G(S.cy)[a], cz = G(S.cz)[a];
-        const size_t at = (size_t)cell_id(S, cx, cy, cz) * FCAP + G(F.slot)[a];
+        const size_t at = (size_t)cell_id(S, cx, cy, cz) * F.cap + G(F.slot)[a];
         const double dt = Te - G(F.t0)[a];
         fvec4 r0;
         r0.x = (float)((G(S.x)[a] + G(S.vx)[a] * dt) - cx * S.cellsize[1]);
@@ -2109,9 +2111,22 @@ int sdmd_fast_open(sdmd_gpu_ctx *c, const sdmd_gpu_opts *o) {
         F.key1 = (unsigned int)(seed >> 32) ^ (0x9E3779B9u * (unsigned int)(s + 1));
         F.validate = fs->validate;
         F.max_batch = o ? o->max_batch : 0;
-        // cell arrays from the reference's linked lists (LIFO order kept)
+        // cell arrays from the reference's linked lists (LIFO order kept),
+        // with room for twice the fullest cell (cells fill and empty as the
+        // atoms move; a cell past its capacity stops the run with an error)
+        int fullest = 0;
+        for (long cell = 1; cell <= ncell; cell++) {
+            int n = 0;
+            for (int a = d->celllist[N + cell]; a && n <= N; a = d->celllist[a]) n++;
+            if (n > N) return SDMD_EINVAL;                 // a cycle in the list
+            fullest = n > fullest ? n : fullest;
+        }
+        int cap = FCAP_MIN;
+        while (cap < 2 * fullest && cap < FCAP_MAX) cap *= 2;
+        if (fullest > cap) return SDMD_EUNSUPPORTED;
+        F.cap = cap;
         int *cnt = (int *)calloc((size_t)ncell + 1, sizeof(int));
-        int *mem = (int *)calloc(((size_t)ncell + 1) * FCAP, sizeof(int));
+        int *mem = (int *)calloc(((size_t)ncell + 1) * cap, sizeof(int));
         int *slot = (int *)calloc((size_t)N + 1, sizeof(int));
         double *t0 = (double *)malloc(sizeof(double) * ((size_t)N + 1));
         if (!cnt || !mem || !slot || !t0) { free(cnt); free(mem); free(slot); free(t0); return SDMD_ENOMEM; }
@@ -2119,19 +2134,19 @@ int sdmd_fast_open(sdmd_gpu_ctx *c, const sdmd_gpu_opts *o) {
         for (long cell = 1; cell <= ncell && rc == SDMD_OK; cell++) {
             int guard = 0;
             for (int a = d->celllist[N + cell]; a; a = d->celllist[a]) {
-                if (++guard > N || cnt[cell] >= FCAP) { rc = SDMD_EUNSUPPORTED; break; }
-                mem[cell * FCAP + cnt[cell]] = a;
+                if (++guard > N || cnt[cell] >= cap) { rc = SDMD_EUNSUPPORTED; break; }
+                mem[cell * cap + cnt[cell]] = a;
                 slot[a] = cnt[cell]++;
             }
         }
         for (int a = 0; a <= N; a++) t0[a] = d->currenttime;
         if (rc == SDMD_OK) rc = fdev_alloc(c, &F.ccnt, cnt, (size_t)ncell + 1);
-        if (rc == SDMD_OK) rc = fdev_alloc(c, &F.cmem, mem, ((size_t)ncell + 1) * FCAP);
+        if (rc == SDMD_OK) rc = fdev_alloc(c, &F.cmem, mem, ((size_t)ncell + 1) * cap);
         if (rc == SDMD_OK) rc = fdev_alloc(c, &F.slot, slot, (size_t)N + 1);
         if (rc == SDMD_OK) rc = fdev_alloc(c, &F.t0, t0, (size_t)N + 1);
         free(cnt); free(mem); free(slot); free(t0);
         if (rc) return rc;
-        if ((rc = fdev_alloc(c, &F.rec, (const fvec4 *)nullptr, ((size_t)ncell + 1) * FCAP * 2))) return rc;
+        if ((rc = fdev_alloc(c, &F.rec, (const fvec4 *)nullptr, ((size_t)ncell + 1) * cap * 2))) return rc;
         if ((rc = fdev_alloc(c, &F.b1, (const double *)nullptr, (size_t)F.nblk))) return rc;
         if ((rc = fdev_alloc(c, &F.b2, (const double *)nullptr, (size_t)F.nblk))) return rc;
         if ((rc = fdev_alloc(c, &F.ba, (const int *)nullptr, (size_t)F.nblk))) return rc;
@@ -2371,14 +2386,14 @@ int sdmd_fast_download(sdmd_gpu_ctx *c, int s, sdmd_system_desc *out) {
     double *t0 = (double *)malloc(sizeof(double) * N1);
     double *b1 = (double *)malloc(sizeof(double) * (size_t)F.nblk);
     int *cnt = (int *)malloc(sizeof(int) * ((size_t)F.ncell + 1));
-    int *mem = (int *)malloc(sizeof(int) * ((size_t)F.ncell + 1) * FCAP);
+    int *mem = (int *)malloc(sizeof(int) * ((size_t)F.ncell + 1) * F.cap);
     if (!t0 || !b1 || !cnt || !mem) { free(t0); free(b1); free(cnt); free(mem); return SDMD_ENOMEM; }
     int rc = SDMD_OK;
 #define FDOWN(dst, src, n) if (rc == SDMD_OK && hipMemcpy(dst, src, sizeof(*(dst)) * (n), hipMemcpyDeviceToHost) != hipSuccess) rc = SDMD_EDEVICE
     FDOWN(t0, F.t0, N1);
     FDOWN(b1, F.b1, (size_t)F.nblk);
     FDOWN(cnt, F.ccnt, (size_t)F.ncell + 1);
-    FDOWN(mem, F.cmem, ((size_t)F.ncell + 1) * FCAP);
+    FDOWN(mem, F.cmem, ((size_t)F.ncell + 1) * F.cap);
     FDOWN(out->x, S.x, N1); FDOWN(out->y, S.y, N1); FDOWN(out->z, S.z, N1);
     FDOWN(out->vx, S.vx, N1); FDOWN(out->vy, S.vy, N1); FDOWN(out->vz, S.vz, N1);
     FDOWN(out->type, S.type, N1);
@@ -2406,7 +2421,7 @@ int sdmd_fast_download(sdmd_gpu_ctx *c, int s, sdmd_system_desc *out) {
         for (long cell = 1; cell <= F.ncell; cell++) {
             int head = 0;
             for (int q = cnt[cell] - 1; q >= 0; q--) {
-                const int a = mem[cell * FCAP + q];
+                const int a = mem[cell * F.cap + q];
                 out->celllist[a] = head;
                 head = a;
             }
@@ -2457,9 +2472,9 @@ __global__ __launch_bounds__(64 * W) void k_fast_energy(FSys *systems, double *o
                 else if (cc[k] >= S.cellnum[k + 1]) { cc[k] = 0; sh[k] = S.box[k + 1]; }
             }
             const int cell = cell_id(S, cc[0], cc[1], cc[2]);
-            const int cnt = min(G(F.ccnt)[cell], FCAP);
+            const int cnt = min(G(F.ccnt)[cell], F.cap);
             for (int q = 0; q < cnt; q++) {
-                const int j = G(F.cmem)[(size_t)cell * FCAP + q];
+                const int j = G(F.cmem)[(size_t)cell * F.cap + q];
                 if (j <= i || j > N) continue;
                 const double dtj = T - G(F.t0)[j];
                 const double vj[3] = {G(S.vx)[j], G(S.vy)[j], G(S.vz)[j]};

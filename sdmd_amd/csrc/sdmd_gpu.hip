@@ -51,7 +51,7 @@ struct Smem {
     int cellbuf[27 * CELL_CAP];
     int cellcnt[27];
     int celloff[28];
-    int ncand;
+    int ncand, over;
     int row_j[ROWCAP];
     int row_b[ROWCAP];
     int dyn_j[SDMD_DYNSLOTS];
@@ -422,15 +422,32 @@ __device__ void predict_atom(Sys &S, Smem &sm, int i) {
     if (tid < min(nb, MAXBOND)) eval_bond(S, sm, tid, &err);
     const int gwave = (B >= 128) ? 1 : 0;
     const int glane = (B >= 128) ? lane : lane - 32;
-    if (wave == gwave && glane >= 0 && glane < 27) {
+    const bool gatherer = wave == gwave && glane >= 0 && glane < 27;
+    if (gatherer) {
         const int c = glane;
         int j = G(S.celllist)[S.N + cell_index(S, sm.A.cx, sm.A.cy, sm.A.cz, c)];
         int cnt = 0;
-        while (j && cnt <= CELL_CAP) {      // bounded walk: an over-full cell is an error
+        while (j && cnt <= CELL_CAP) {      // bounded walk: a fuller cell takes the windowed gather
             if (cnt < CELL_CAP) sm.cellbuf[c * CELL_CAP + cnt] = j;
             cnt++;
             j = G(S.celllist)[j];
         }
+        sm.cellcnt[c] = cnt;
+    }
+    if (tid == 0) sm.over = 0;
+    __syncthreads();
+    if (gatherer && sm.cellcnt[glane] > CELL_CAP) {
+        // a cell holds more atoms than the staging area: count every cell's
+        // list in full (the lists are the reference's, Events.c:158-200)
+        sm.over = 1;
+    }
+    __syncthreads();
+    const bool over = sm.over;
+    if (over && gatherer) {
+        const int c = glane;
+        int j = G(S.celllist)[S.N + cell_index(S, sm.A.cx, sm.A.cy, sm.A.cz, c)];
+        int cnt = 0;
+        while (j && cnt <= S.N) { cnt++; j = G(S.celllist)[j]; }
         sm.cellcnt[c] = cnt;
     }
     __syncthreads();
@@ -444,9 +461,9 @@ __device__ void predict_atom(Sys &S, Smem &sm, int i) {
             if (inh) take_inherited(sm.A, S, sm.bj[win], SDMD_BOND);
             else take_computed(sm.A, S, sm.bt[win], sm.bpot[win], sm.bsub[win], SDMD_BOND, sm.bj[win]);
         }
-        // prefix sum of the cell counts
-        int c = lane < 27 ? min(sm.cellcnt[lane], CELL_CAP) : 0;
-        if (lane < 27 && sm.cellcnt[lane] > CELL_CAP) sm.err = ERR_CELLCAP;
+        // prefix sum of the cell counts (the candidates' order: cells in
+        // DMD.c:117-121 order, each in list order)
+        int c = lane < 27 ? sm.cellcnt[lane] : 0;
         int incl = c;
 #pragma unroll
         for (int off = 1; off < 32; off <<= 1) {
@@ -456,75 +473,108 @@ __device__ void predict_atom(Sys &S, Smem &sm, int i) {
         if (lane < 27) sm.celloff[lane + 1] = incl;
         if (lane == 0) sm.celloff[0] = 0;
         if (lane == 26) sm.ncand = incl;
+        if (lane < 27 && sm.cellcnt[lane] > S.N) sm.err = ERR_LINKLIST;
     }
     __syncthreads();
-    const int ncand = sm.ncand;
-    for (int k = tid; k < 27 * CELL_CAP; k += B) {
-        const int c = k / CELL_CAP, q = k % CELL_CAP;
-        if (q < sm.cellcnt[c]) sm.cand_j[sm.celloff[c] + q] = sm.cellbuf[k];
-    }
-    __syncthreads();
+    const int ntot = sm.err ? 0 : sm.ncand;         // a broken list: no scan (the error ends the run)
+    const int wsz = S.win;                            // candidates per pass, <= MAXC
+    // the staged cells serve a single pass; a fuller cell or a longer list
+    // re-walks the lists for each window
+    const bool rewalk = over || ntot > wsz;
+    // the candidates of [w0, w0 + wsz) of the ordered list into cand_j
+    auto gather_window = [&](int w0) {
+        if (!rewalk) {
+            for (int k = tid; k < 27 * CELL_CAP; k += B) {
+                const int c = k / CELL_CAP, q = k % CELL_CAP;
+                if (q < sm.cellcnt[c]) sm.cand_j[sm.celloff[c] + q] = sm.cellbuf[k];
+            }
+        } else if (gatherer) {
+            const int c = glane;
+            const int w1 = w0 + wsz;
+            int g = sm.celloff[c];
+            if (g < w1 && sm.celloff[c + 1] > w0) {
+                int j = G(S.celllist)[S.N + cell_index(S, sm.A.cx, sm.A.cy, sm.A.cz, c)];
+                for (; j && g < w1; g++, j = G(S.celllist)[j])
+                    if (g >= w0) sm.cand_j[g - w0] = j;
+            }
+        }
+        __syncthreads();
+    };
     PROF_MARK(S, PH_BONDSEL, pt);
     double calib = S.cutoffr * SDMD_EXTEND_RATIO;
     calib *= calib;
     int my_cand = 0, my_eval = 0;
-    for (int k = tid; k < ncand; k += B) {
-        eval_collision(S, sm, k, calib, &err);
-        my_cand += (sm.cand_j[k] != i);
-        my_eval += (sm.cand_fl[k] >> 1) & 1;
-    }
     if (tid < min(nb, MAXBOND)) my_eval++;
-    if (my_cand) atomicAdd(&sm.cnt_cand, my_cand);
-    if (my_eval) atomicAdd(&sm.cnt_eval, my_eval);
-    __syncthreads();
-    PROF_MARK(S, PH_EVAL, pt);
-    if (wave == 0) {
-        double m = sm.A.t;
-        int win = -1, inh = 0;
-        wave_select(ncand, sm.cand_t, sm.cand_fl, sm.cand_j, S.evt, m, win, inh);
-        if (lane == 0) {
-            if (win >= 0) {
-                if (inh) take_inherited(sm.A, S, sm.cand_j[win], SDMD_COLI);
-                else take_computed(sm.A, S, sm.cand_t[win], sm.cand_pot[win], sm.cand_sub[win], SDMD_COLI, sm.cand_j[win]);
-            }
-            // PBCandCrossCellTime, ThermostatTime
-            pcc_time(S, sm.A);
-            if (S.thermostat) {
-                double z = next_normal(S, sm);
-                double t = S.thermoF * ((z < 0) ? (-z) : (z));
-                job_assign_atom(sm.A, S, t, 0, SDMD_INVALID_INTER, SDMD_THER, 0);
-            }
-            // HBTime head, TimePrediction.c:309-330
-            sm.hbscan = 0;
-            const int i0 = sm.A.i;
-            if (G(S.hbitype)[i0] == SDMD_HBSKIP) {
-                G(S.hbitype)[i0] = SDMD_NOEVENT;
-            } else {
-                G(S.hbitype)[i0] = SDMD_NOEVENT;
-                if (sm.A.hbbond > 0) hb_time_bonded(S, sm.A, &err);
-                else if (sm.A.role == 'A' || sm.A.role == 'D' || sm.A.role == 'S') sm.hbscan = 1;
-            }
-        }
-    }
-    __syncthreads();
-    PROF_MARK(S, PH_SELECT, pt);
-    if (sm.hbscan) {
-        int my_hb = 0;
+    // CollisionTime over the ordered candidates, window by window: the
+    // running minimum (and CheckDuplication against it) carries over, so the
+    // result is that of one sequential pass
+    for (int w0 = 0; w0 < ntot || w0 == 0; w0 += wsz) {
+        const int ncand = min(ntot - w0, wsz);
+        gather_window(w0);
         for (int k = tid; k < ncand; k += B) {
-            eval_hb(S, sm, k, calib, &err);
-            my_hb += (sm.cand_fl[k] >> 1) & 1;
+            eval_collision(S, sm, k, calib, &err);
+            my_cand += (sm.cand_j[k] != i);
+            my_eval += (sm.cand_fl[k] >> 1) & 1;
         }
-        if (my_hb) atomicAdd(&sm.cnt_eval, my_hb);
         __syncthreads();
         if (wave == 0) {
             double m = sm.A.t;
             int win = -1, inh = 0;
             wave_select(ncand, sm.cand_t, sm.cand_fl, sm.cand_j, S.evt, m, win, inh);
             if (lane == 0 && win >= 0) {
-                if (inh) take_inherited(sm.A, S, sm.cand_j[win], SDMD_HB);
-                else take_computed(sm.A, S, sm.cand_t[win], sm.cand_pot[win], SDMD_TBDHBFORM, SDMD_HB, sm.cand_j[win]);
+                if (inh) take_inherited(sm.A, S, sm.cand_j[win], SDMD_COLI);
+                else take_computed(sm.A, S, sm.cand_t[win], sm.cand_pot[win], sm.cand_sub[win], SDMD_COLI, sm.cand_j[win]);
             }
         }
+        __syncthreads();
+    }
+    if (my_cand) atomicAdd(&sm.cnt_cand, my_cand);
+    if (my_eval) atomicAdd(&sm.cnt_eval, my_eval);
+    PROF_MARK(S, PH_EVAL, pt);
+    if (tid == 0) {
+        // PBCandCrossCellTime, ThermostatTime
+        pcc_time(S, sm.A);
+        if (S.thermostat) {
+            double z = next_normal(S, sm);
+            double t = S.thermoF * ((z < 0) ? (-z) : (z));
+            job_assign_atom(sm.A, S, t, 0, SDMD_INVALID_INTER, SDMD_THER, 0);
+        }
+        // HBTime head, TimePrediction.c:309-330
+        sm.hbscan = 0;
+        const int i0 = sm.A.i;
+        if (G(S.hbitype)[i0] == SDMD_HBSKIP) {
+            G(S.hbitype)[i0] = SDMD_NOEVENT;
+        } else {
+            G(S.hbitype)[i0] = SDMD_NOEVENT;
+            if (sm.A.hbbond > 0) hb_time_bonded(S, sm.A, &err);
+            else if (sm.A.role == 'A' || sm.A.role == 'D' || sm.A.role == 'S') sm.hbscan = 1;
+        }
+    }
+    __syncthreads();
+    PROF_MARK(S, PH_SELECT, pt);
+    if (sm.hbscan) {
+        // the free-atom H-bond scan over the same ordered candidates
+        int my_hb = 0;
+        for (int w0 = 0; w0 < ntot; w0 += wsz) {
+            const int ncand = min(ntot - w0, wsz);
+            if (rewalk) gather_window(w0);
+            for (int k = tid; k < ncand; k += B) {
+                eval_hb(S, sm, k, calib, &err);
+                my_hb += (sm.cand_fl[k] >> 1) & 1;
+            }
+            __syncthreads();
+            if (wave == 0) {
+                double m = sm.A.t;
+                int win = -1, inh = 0;
+                wave_select(ncand, sm.cand_t, sm.cand_fl, sm.cand_j, S.evt, m, win, inh);
+                if (lane == 0 && win >= 0) {
+                    if (inh) take_inherited(sm.A, S, sm.cand_j[win], SDMD_HB);
+                    else take_computed(sm.A, S, sm.cand_t[win], sm.cand_pot[win], SDMD_TBDHBFORM, SDMD_HB, sm.cand_j[win]);
+                }
+            }
+            __syncthreads();
+        }
+        if (my_hb) atomicAdd(&sm.cnt_eval, my_hb);
     }
     PROF_MARK(S, PH_HB, pt);
     if (tid == 0) {
@@ -1001,6 +1051,7 @@ static int upload_system(sdmd_gpu_ctx *c, Sys *S, const sdmd_system_desc *d, con
     UP(sacc, d->step_acc, (size_t)d->nstep_pool + 1);
     UP(node, (const int *)nullptr, 2 * N1);
     UP(leaf, (const int *)nullptr, N1 + 1);
+    S->win = (o && o->window > 0) ? o->window : MAXC;
     if (o && o->trace) {
         S->trace_cap = o->trace_capacity > 0 ? o->trace_capacity : 100000;
         UP(trace, (const sdmd_gpu_pop_rec *)nullptr, (size_t)S->trace_cap);
@@ -1038,6 +1089,7 @@ extern "C" int sdmd_gpu_open(const sdmd_system_desc *const *systems, int32_t nsy
     const int mode = opts ? opts->mode : SDMD_MODE_PLAIN;
     if (mode != SDMD_MODE_PLAIN && mode != SDMD_MODE_FAST) return SDMD_EUNSUPPORTED;
     if (mode == SDMD_MODE_FAST && block == 128) return SDMD_EINVAL;
+    if (opts && (opts->window < 0 || opts->window > MAXC)) return SDMD_EINVAL;
     for (int s = 0; s < nsys; s++) {
         const sdmd_system_desc *d = systems[s];
         if (!d || d->natom < 2) return SDMD_EINVAL;

@@ -111,21 +111,24 @@ class GpuEngine:
                   independent events (one per wavefront); `validate` re-solves
                   every collision candidate in FP64 and counts agreement;
                   `helpers` workgroups per system take re-predictions on
-                  other CUs (0: as many as fit, -1: none)."""
+                  other CUs (0: as many as fit, -1: none).
+    `window` (plain): neighbour candidates per pass of a prediction's
+    windowed scan (0: 1728); a smaller window takes more passes and gives
+    the same result."""
 
     PHASES = ["pop", "sweep", "event", "stage", "gather", "bondsel", "eval", "select", "hb", "tail", "tree"]
     FAST_PHASES = ["select", "exec", "stage", "screen", "confirm", "other", "tree", "event", "bond", "scan", "collect"]
 
     def __init__(self, systems, block=256, trace=False, trace_capacity=0, hash_every=0,
                  normals_per_pop=5.0, max_pops=0, profile=False, mode="plain", validate=False,
-                 max_batch=0, seed=0, helpers=0):
+                 max_batch=0, seed=0, helpers=0, window=0):
         self.systems = list(systems)
         self.mode = mode
         L = _lib.lib()
         self._descs = [s.desc() for s in self.systems]
         arr = (C.POINTER(SystemDesc) * len(self._descs))(*[C.pointer(d) for d in self._descs])
         opts = GpuOpts(MODES[mode], block, 1 if trace else 0, 1 if profile else 0, trace_capacity, hash_every,
-                       1 if validate else 0, max_batch, seed, helpers, 0)
+                       1 if validate else 0, max_batch, seed, helpers, window)
         h = C.c_void_p()
         check(L.sdmd_gpu_open(arr, len(self._descs), C.byref(opts), C.byref(h)), "sdmd_gpu_open")
         self._h = h

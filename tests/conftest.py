@@ -11,7 +11,7 @@ for p in (ROOT, HERE, GOLDEN):
     if p not in sys.path:
         sys.path.insert(0, p)
 
-CASES = ["ala10ext", "ala10helix", "mix4x20", "ala10nve", "ala10cyl", "cyl8x20"]
+CASES = ["ala10ext", "ala10helix", "mix4x20", "ala10nve", "ala10cyl", "cyl8x20", "dense30"]
 
 
 def pytest_configure(config):

@@ -51,7 +51,12 @@ CASES = {
                  100000, 5000),
     "cyl8x20": ("cyl8x20", dict(temperature=0.5962, cutoff=7.25, wall="smooth", wall_type="cylinder"),
                 300000, 3000),
+    # 76 atoms in the fullest 12 A cell: the windowed neighbour scan
+    "dense30": ("dense", dict(temperature=0.5962, cutoff=12.0), 100000, 3000),
 }
+
+
+DENSE_SEQ = ["TRP", "ARG", "LYS", "PHE", "TYR"] * 6
 
 
 def build_ref():
@@ -67,6 +72,8 @@ def make_input(kind, d, params):
         builder.write_input_dir(builder.ala10_cylinder(seed=1), d, **params)
     elif kind == "cyl8x20":
         builder.write_input_dir(builder.cylinder_peptides(8, 20, seed=5, radius=40.0, box_x=160.0, margin=4.0), d, **params)
+    elif kind == "dense":
+        builder.write_input_dir(builder.single_chain(DENSE_SEQ, seed=8, box=72.0), d, **params)
     elif kind == "helix":
         os.makedirs(d, exist_ok=True)
         shutil.copy(os.path.join(REF, "input", "ala10.gro"), os.path.join(d, "coordinate.gro"))

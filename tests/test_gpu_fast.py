@@ -257,3 +257,46 @@ def test_fast_helper_workgroups_keep_the_trajectory(require_gpu, mid_nvt):
             assert stats[k]["pair_evals"] == base_stats[k]["pair_evals"]
             for f in FIELDS:
                 assert np.array_equal(states[k].arr[f], base_states[k].arr[f]), (k, f)
+
+
+@pytest.fixture(scope="module")
+def dense_nve(tmp_path_factory):
+    """The dense single chain (76 atoms in the fullest 12 A cell), NVE."""
+    from sdmd_amd import builder
+    from sdmd_amd.engine import System
+
+    d = tmp_path_factory.mktemp("densenve")
+    builder.write_input_dir(builder.single_chain(["TRP", "ARG", "LYS", "PHE", "TYR"] * 6, seed=8, box=72.0), str(d), cutoff=12.0,
+                            thermostat="no")
+    return System.load(str(d))
+
+
+def test_fast_dense_cells(require_gpu, dense_nve):
+    """Cells fuller than 64 atoms: the fast engine sizes its cell slots from
+    the fullest cell; bulk prediction matches the oracle, screened
+    predictions agree with the full FP64 scan, cells track positions."""
+    from sdmd_amd.engine import GpuEngine
+
+    _bulk_check(dense_nve)
+    eng = GpuEngine([dense_nve], block=512, mode="fast", validate=True)
+    eng.prepare()
+    eng.run(20000)
+    fs, st = eng.fast_stats(0), eng.stats(0)
+    assert st["error"] == 0
+    assert fs["validated"] > 20000 and fs["agree"] == fs["validated"], fs
+    a = eng.state(0).arr
+    eng.close()
+    cs = np.array(dense_nve.arrays.scalars["cellsize"][:3])
+    pos = np.stack([a["x"][1:], a["y"][1:], a["z"][1:]], 1)
+    cell = np.stack([a["cellx"][1:], a["celly"][1:], a["cellz"][1:]], 1)
+    lo = cell * cs
+    assert np.all(pos >= lo - 1e-6) and np.all(pos < lo + cs + 1e-6)
+    N = dense_nve.natom
+    cl = a["celllist"]
+    counts = []
+    for c in range(1, len(cl) - N):
+        n, j = 0, cl[N + c]
+        while j:
+            n, j = n + 1, cl[j]
+        counts.append(n)
+    assert sum(counts) == N and max(counts) > 64

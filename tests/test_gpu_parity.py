@@ -89,3 +89,43 @@ def test_event_sequence_matches_reference(require_gpu, name):
     assert np.array_equal(st["per_type"][: len(counts)], counts)
     assert st["recal"] == int(g["trace"]["recal"])
     eng.close()
+
+
+@pytest.mark.parametrize("name,window", [("dense30", 64), ("dense30", 5), ("ala10helix", 7)])
+def test_windowed_scan_matches_reference(require_gpu, name, window):
+    """The neighbour scan in windows of `window` candidates (the staged
+    cells bypassed, the cell lists re-walked per window, the running minimum
+    and CheckDuplication carried across windows) pops the reference's
+    events: full records for the stored prefix and the state hashes.
+    dense30 holds 76 atoms in its fullest cell, more than the 64 staged per
+    neighbour cell, so its default run takes the same path."""
+    from sdmd_amd.engine import GpuEngine
+
+    sysm, g = golden_system(name)
+    nfull = len(g["trace"]["target"])
+    npops = 20000
+    eng = GpuEngine([sysm], trace=True, trace_capacity=npops, hash_every=1000, max_pops=npops, window=window)
+    eng.prepare()
+    st = eng.state(0)
+    for f in EVENT_FIELDS:
+        assert np.array_equal(st.arr[f][1:], g["setup"][f][1:].astype(st.arr[f].dtype)), f
+    eng.run(npops)
+    assert eng.stats(0)["error"] == 0
+    tr = eng.trace(0)
+    assert first_mismatch(tr, g, nfull) is None
+    for f in ("target", "partner", "type", "exec"):
+        assert np.array_equal(tr[f], g["trace"]["seq_" + f][:npops].astype(tr[f].dtype)), f
+    hs = eng.hashes(0)
+    ref = g["trace"]["hashes"]
+    assert np.array_equal(hs[:, 1], ref[: len(hs), 1])
+    eng.close()
+
+
+def test_window_option_is_checked(require_gpu):
+    from sdmd_amd._lib import SdmdError
+    from sdmd_amd.engine import GpuEngine
+
+    sysm, _ = golden_system("ala10ext")
+    for bad in (-1, 27 * 64 + 1):
+        with pytest.raises(SdmdError):
+            GpuEngine([sysm], window=bad)

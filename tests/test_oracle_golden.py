@@ -70,3 +70,24 @@ def test_config1_golden_run_covers_1e5_events():
 def test_nve_run_has_no_thermostat_events():
     _, g = golden_system("ala10nve")
     assert g["trace"]["counts_type"][4] == 0
+
+
+def cell_counts(setup, N):
+    """Atoms per cell from a cell list (heads at N + cell, Events.c:158-200)."""
+    cl = setup["celllist"]
+    out = []
+    for head in cl[N + 1:]:
+        n, j = 0, int(head)
+        while j:
+            n, j = n + 1, int(cl[j])
+        out.append(n)
+    return np.array(out)
+
+
+def test_dense_case_fills_a_cell_beyond_the_staging_area():
+    """dense30 puts more atoms in one cell than the plain engine stages per
+    neighbour cell (64), so its predictions take the windowed scan."""
+    sysm, g = golden_system("dense30")
+    counts = cell_counts(g["setup"], sysm.natom)
+    assert counts.sum() == sysm.natom
+    assert counts.max() > 64, counts.max()
