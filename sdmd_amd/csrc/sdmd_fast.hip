@@ -247,8 +247,17 @@ constexpr int FCONSTEP = 8;       // steps per staged constraint
 
 struct FWave {
     FAtom a;                     // the atom being predicted (uniform across the wave)
-    // its bonded partners' states (lanes 32..47 stage bond q = lane - 32)
-    double bxj[16], byj[16], bzj[16], bvx[16], bvy[16], bvz[16], bt0[16], blo[16], bhi[16];
+    union {
+        // its bonded partners' states (lanes 32..47 stage bond q = lane - 32)
+        struct {
+            double bxj[16], byj[16], bzj[16], bvx[16], bvy[16], bvz[16], bt0[16], blo[16], bhi[16];
+        };
+        // after the bond phase: the free H-bond scan's candidates that pass
+        // its cheap conditions (atom id and candidate index), 128 at a time
+        struct {
+            int hbl_j[128], hbl_k[128];
+        };
+    };
     long long bcj[16];
     int bjj[16];
     int row_j[FROW], row_b[FROW];
@@ -1023,19 +1032,26 @@ __device__ double bond64(const FSys &F, const FAtom &A, int e, int &sub) {
     return (t > 0) ? t : __builtin_inf();
 }
 
-// free-atom HBTime candidate (TimePrediction.c:414-478) at local time
-__device__ double hbfree64(const FSys &F, const FAtom &A, int j, double calib, double &pot, bool &reached, int *err) {
+// free-atom HBTime candidate (TimePrediction.c:414-478) at local time, in
+// two parts: the conditions on j's bookkeeping alone (role pairing, j not
+// H-bonded, sequence separation within a chain: TimePrediction.c:420-430),
+// then distance, CheckPairMatch, FindPair and the contact time
+__device__ __forceinline__ bool hbfree_pass(const FSys &F, const FAtom &A, int j) {
     const Sys &S = F.s;
-    reached = false;
-    pot = 0;
+    if (j < 1 || j > S.N || j == A.i) return false;
     const signed char ri = A.role, rj = G(S.role)[j];
     const bool pairok = (ri == 'A' && rj == 'D') || (ri == 'D' && rj == 'A') ||
                         (ri == 'S' && G(S.type)[j] == 2) || (rj == 'S' && A.type == 2) || (ri == 'S' && rj == 'S');
     int sep = A.aanum - G(S.aanum)[j];
     if (sep < 0) sep = -sep;
     const int pj = G(S.protein)[j];
-    if (!(j != A.i && pairok && G(S.hbbond)[j] == 0 && ((A.protein == pj && sep >= 4) || A.protein != pj)))
-        return __builtin_inf();
+    return pairok && G(S.hbbond)[j] == 0 && ((A.protein == pj && sep >= 4) || A.protein != pj);
+}
+
+__device__ double hbfree64(const FSys &F, const FAtom &A, int j, double calib, double &pot, bool &reached, int *err) {
+    const Sys &S = F.s;
+    reached = false;
+    pot = 0;
     Kin k;
     kin_local(F, A, j, k);
     if (!(k.rr <= calib) || check_pair_match(S, A.i, j, err) < 0) return __builtin_inf();
@@ -1154,22 +1170,52 @@ __device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm
             take(best, S, tt, pot, sub, SDMD_HB, j);
             nfp64 += (lane == 0) ? 1 : 0;
         } else if (A.role == 'A' || A.role == 'D' || A.role == 'S') {
-            // free HB atom: scan the neighbourhood in FP64
+            // free HB atom: scan the neighbourhood in FP64, 128 candidates at
+            // a time: the conditions on j's bookkeeping for two candidates
+            // per lane (their loads issued together), then the FP64 part for
+            // the candidates that pass, packed onto the wave's lanes in
+            // candidate order (most candidates fail the role pairing)
             const int n = wm.coff[27];
             Best mine = best;
             mine.t = __builtin_inf();
             int mink = 0x7fffffff;
             int my = 0;
-            for (int k = lane; k < n; k += 64) {
-                const int c = cand_cell(wm, k);
-                const size_t at = (size_t)wm.cell[c] * F.cap + (k - wm.coff[c]);
-                const int j = __float_as_int(G(F.rec)[2 * at + 1].w) >> 8;
-                if (j < 1 || j > S.N) continue;
-                double pot;
-                bool reached;
-                const double tt = hbfree64(F, A, j, calib, pot, reached, err);
-                my += reached;
-                if (tt < mine.t) { mine.t = tt; mine.pot = pot; mine.partner = j; mink = k; }
+            for (int base = 0; base < n; base += 128) {
+                int jj[2];
+                bool ok[2];
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const int k = base + 64 * u + lane;
+                    jj[u] = 0;
+                    if (k < n) {
+                        const int c = cand_cell(wm, k);
+                        const size_t at = (size_t)wm.cell[c] * F.cap + (k - wm.coff[c]);
+                        jj[u] = __float_as_int(G(F.rec)[2 * at + 1].w) >> 8;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 2; u++) ok[u] = hbfree_pass(F, A, jj[u]);
+                const unsigned long long m0 = __ballot(ok[0]), m1 = __ballot(ok[1]);
+                const int n0 = __popcll(m0), npass = n0 + __popcll(m1);
+                const unsigned int lo = (unsigned int)((1ull << lane) - 1);
+                const unsigned int hi = (unsigned int)(((1ull << lane) - 1) >> 32);
+                const int r0 = __builtin_amdgcn_mbcnt_hi((unsigned int)(m0 >> 32) & hi,
+                                                         __builtin_amdgcn_mbcnt_lo((unsigned int)m0 & lo, 0));
+                const int r1 = n0 + __builtin_amdgcn_mbcnt_hi((unsigned int)(m1 >> 32) & hi,
+                                                              __builtin_amdgcn_mbcnt_lo((unsigned int)m1 & lo, 0));
+                __builtin_amdgcn_wave_barrier();
+                if (ok[0]) { wm.hbl_j[r0] = jj[0]; wm.hbl_k[r0] = base + lane; }
+                if (ok[1]) { wm.hbl_j[r1] = jj[1]; wm.hbl_k[r1] = base + 64 + lane; }
+                __builtin_amdgcn_wave_barrier();
+                for (int r = lane; r < npass; r += 64) {
+                    const int j = wm.hbl_j[r], k = wm.hbl_k[r];
+                    double pot;
+                    bool reached;
+                    const double tt = hbfree64(F, A, j, calib, pot, reached, err);
+                    my += reached;
+                    if (tt < mine.t) { mine.t = tt; mine.pot = pot; mine.partner = j; mink = k; }
+                }
+                __builtin_amdgcn_wave_barrier();
             }
             nfp64 += wave_sum_i(my);
             double tw = mine.t;
@@ -1598,7 +1644,7 @@ __global__ __launch_bounds__(64 * W) void k_fast_prepare(FSys *systems) {
 // A helper workgroup: its wavefronts claim the owner's published jobs
 // (re-predictions) until the owner stops, then add their statistics to hp.
 template <int W, int FQ>
-__device__ void helper_loop(FSys &F, const ScrTab &tab, FShared &sh, FJobs *J, FHelp *hp, int xcc) {
+__device__ __forceinline__ void helper_loop(FSys &F, const ScrTab &tab, FShared &sh, FJobs *J, FHelp *hp, int xcc) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     WStats ws = {0, 0, 0, 0};
     for (;;) {
@@ -1921,17 +1967,7 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, FJobs *jobs,
         const int nren = nheavy + nlight;             // <= 2 * nex <= 2 * FSEL: the two ends do not meet
         auto ren_at = [&](int r) { return r < nheavy ? r : 2 * FSEL - nlight + (r - nheavy); };
         err = 0;
-        if (H == 0) {
-            for (;;) {
-                int r = 0;
-                if (lane == 0) r = atomicAdd(&sh.next_ren, 1);
-                r = __shfl(r, 0, 64);
-                if (r >= nren || sh.err != 0) break;
-                const int q = ren_at(r);
-                predict_fast<FQ>(F, tab, sh.w[wave], sh.ren_a[q], sh.ren_t[q], ws, &err);
-                if (__ballot(err != 0)) break;          // uniform across the wave
-            }
-        } else {
+        if (H > 0) {
             // publish the round's jobs, then claim them alongside the helpers
             for (int r = tid; r < nren; r += 64 * W) {
                 const int q = ren_at(r);
@@ -1951,16 +1987,27 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, FJobs *jobs,
                 __hip_atomic_store(&J->word, (unsigned long long)nren << 32, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             __syncthreads();
-            for (;;) {
-                int r = 0;
-                if (lane == 0) r = sh.err != 0 ? -1 : job_claim(J);
-                r = __shfl(r, 0, 64);
-                if (r < 0) break;
-                const int q = ren_at(r);
-                predict_fast<FQ>(F, tab, sh.w[wave], sh.ren_a[q], sh.ren_t[q], ws, &err);
-                if (lane == 0) job_done(J);
-                if (__ballot(err != 0)) break;          // uniform across the wave
+        }
+        // one prediction call site for both forms: a wave takes the next job
+        // from the LDS counter (no helpers) or from the system's job word
+        for (;;) {
+            int r = -1;
+            if (lane == 0 && sh.err == 0) {
+                if (H == 0) {
+                    r = atomicAdd(&sh.next_ren, 1);
+                    if (r >= nren) r = -1;
+                } else {
+                    r = job_claim(J);
+                }
             }
+            r = __shfl(r, 0, 64);
+            if (r < 0) break;
+            const int q = ren_at(r);
+            predict_fast<FQ>(F, tab, sh.w[wave], sh.ren_a[q], sh.ren_t[q], ws, &err);
+            if (H > 0 && lane == 0) job_done(J);
+            if (__ballot(err != 0)) break;          // uniform across the wave
+        }
+        if (H > 0) {
             if (err) atomicCAS(&sh.err, 0, err);
             if (tid == 0) {
                 // every claimed job completes (its wavefront is running); an
