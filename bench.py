@@ -33,6 +33,7 @@ import argparse
 import json
 import math
 import os
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -108,17 +109,30 @@ def cpu_baseline(dirs, natoms, seconds, procs):
     pick = [order[int(round(q * (len(order) - 1) / max(P - 1, 1)))] for q in range(P)]
     have_taskset = subprocess.run(["which", "taskset"], capture_output=True).returncode == 0
     runs = []
+    barrier = tempfile.mkdtemp(prefix="sdmd_cpu_barrier_")
     for q, k in enumerate(pick):
         out = tempfile.mkdtemp(prefix="sdmd_cpu_")
-        # a pop costs about 12.5 ns per atom (the O(N) position sweep) + 5 us on one core
-        pops = int(seconds / (12.5e-9 * natoms[k] + 5e-6))
-        env = dict(os.environ, SDMD_LIGHT="1", SDMD_DUMP=out, SDMD_MAX_POPS=str(pops))
+        # every loop waits at its start for the others (SDMD_BARRIER) and runs
+        # `seconds` of its event loop (SDMD_MAX_SECONDS); the pop cap is a
+        # safety bound only (a pop costs at least ~1.5 us + 2.4 ns per atom)
+        pops = int(seconds / 1e-6)
+        env = dict(os.environ, SDMD_LIGHT="1", SDMD_DUMP=out, SDMD_MAX_POPS=str(pops),
+                   SDMD_MAX_SECONDS=repr(float(seconds)), SDMD_BARRIER="%s:%d" % (barrier, P))
         cmd = [ref, "-f", "coordinate.gro", "-i", dirs[k], "-o", out]
         if have_taskset:
             cmd = ["taskset", "-c", str(allowed[q])] + cmd
         fh = open(os.path.join(out, "stdout.txt"), "w")
         runs.append((k, out, subprocess.Popen(cmd, env=env, stdout=fh, stderr=subprocess.STDOUT), fh))
     rates, events, t_lo, t_hi, npops = [], 0, [], [], 0
+    # a process that fails would leave the others waiting at the start barrier
+    while any(p.poll() is None for _, _, p, _ in runs):
+        bad = [(k, out) for k, out, p, _ in runs if p.poll() not in (None, 0)]
+        if bad:
+            for _, _, p, _ in runs:
+                if p.poll() is None:
+                    p.kill()
+            raise SystemExit("bench.py: the reference failed on %s (see %s)" % (dirs[bad[0][0]], bad[0][1]))
+        time.sleep(0.05)
     for k, out, p, fh in runs:
         if p.wait() != 0:
             raise SystemExit("bench.py: the reference failed on %s (see %s)" % (dirs[k], out))
@@ -131,11 +145,14 @@ def cpu_baseline(dirs, natoms, seconds, procs):
         t_lo.append(float(v["start"]))
         t_hi.append(float(v["end"]))
     overlap = max(0.0, min(t_hi) - max(t_lo))
+    for d in [barrier] + [out for _, out, _, _ in runs]:
+        shutil.rmtree(d, ignore_errors=True)
     return {"value": sum(rates), "unit": "events/s", "cores": P, "kind": "reference",
             "host_cores": cores, "cpu_model": model,
             "sample": "reference sDMD (own sources, -O2), %d concurrent single-threaded processes pinned to one "
-                      "core each, one batch system each (%d-%d atoms), about %.0f s of each event loop (%d pops, "
-                      "%d events in all); value = sum of the %d loop rates (all loops ran together for %.1f s)"
+                      "core each, one batch system each (%d-%d atoms), started together, %.0f s of each event loop "
+                      "(%d pops, %d events in all); value = sum of the %d loop rates (all loops ran together for "
+                      "%.1f s)"
                       % (P, min(natoms[k] for k in pick), max(natoms[k] for k in pick), seconds, npops, events, P,
                          overlap),
             "per_process_events_per_s": [round(r, 1) for r in rates]}

@@ -13,16 +13,24 @@
  *   DoEvent  (SGThread.c:101)       -> marks the pop as executed
  *   UpdateCBT (SGThread.c:120,149)  -> per-pop trace record incl. the
  *                                      re-predicted atoms; state hashes;
- *                                      stops after SDMD_MAX_POPS pops
+ *                                      stops after SDMD_MAX_POPS pops, or
+ *                                      after SDMD_MAX_SECONDS of event loop
+ *   (loop start, light mode)        -> with SDMD_BARRIER=<dir>:<n>, waits
+ *                                      until n processes have reached their
+ *                                      loop (one file each in <dir>), so
+ *                                      concurrent timing runs start together
  *   SaveData (DataSave.c)           -> disabled (no output files) unless
  *                                      SDMD_SAVE is set; then passed through,
  *                                      with the pop count of every
  *                                      trajectory dump (dumps.txt)
  * Nothing here changes the reference's arithmetic or control flow.
  */
-#define _POSIX_C_SOURCE 199309L
+#define _POSIX_C_SOURCE 200809L
+#include <dirent.h>
 #include <stdint.h>
+#include <string.h>
 #include <time.h>
+#include <unistd.h>
 #include "DMD.h"
 
 int __real_rand(void);
@@ -35,6 +43,7 @@ void __real_UpdateCBT(int *);
 static long long g_rand_calls = 0;
 static long long g_pops = 0;
 static long long g_max_pops = -1;
+static double g_max_seconds = -1;
 static long long g_hash_every = 1000;
 static int g_exec = 0;
 static FILE *g_trace = NULL, *g_hash = NULL;
@@ -192,6 +201,33 @@ static void write_topology(void) {
     fclose(f);
 }
 
+/* SDMD_BARRIER=<dir>:<n>: announce this process in <dir>, then wait (at
+   most 10 minutes) until n processes have announced themselves */
+static void start_barrier(const char *spec) {
+    char dir[900];
+    const char *colon = strrchr(spec, ':');
+    if (!colon || colon == spec || (size_t)(colon - spec) >= sizeof dir) return;
+    memcpy(dir, spec, (size_t)(colon - spec));
+    dir[colon - spec] = 0;
+    const int n = atoi(colon + 1);
+    char path[1024];
+    snprintf(path, sizeof path, "%s/ready.%ld", dir, (long)getpid());
+    FILE *f = fopen(path, "w");
+    if (f) fclose(f);
+    const struct timespec nap = {0, 1000000};
+    for (long w = 0; w < 600000; w++) {
+        int k = 0;
+        DIR *d = opendir(dir);
+        if (d) {
+            struct dirent *de;
+            while ((de = readdir(d))) k += strncmp(de->d_name, "ready.", 6) == 0;
+            closedir(d);
+        }
+        if (k >= n) return;
+        nanosleep(&nap, NULL);
+    }
+}
+
 void __wrap_CreateCBT(void) {
     const char *e = getenv("SDMD_DUMP");
     if (e) snprintf(g_dir, sizeof g_dir, "%s", e);
@@ -216,6 +252,8 @@ void __wrap_CreateCBT(void) {
         g_hash = open_out("hash.bin", "wb");
     }
     if (g_max_pops == 0) { if (g_trace) { fclose(g_trace); fclose(g_hash); } exit(0); }
+    if ((e = getenv("SDMD_MAX_SECONDS"))) g_max_seconds = atof(e);
+    if ((e = getenv("SDMD_BARRIER"))) start_barrier(e);
     clock_gettime(CLOCK_MONOTONIC, &g_t0);
 }
 
@@ -259,6 +297,12 @@ void __wrap_UpdateCBT(int *renew) {
     if (g_hash && g_hash_every > 0 && g_pops % g_hash_every == 0) {
         int64_t rec[2] = {g_pops, (int64_t)state_hash()};
         fwrite(rec, sizeof rec, 1, g_hash);
+    }
+    if (g_max_seconds > 0 && (g_pops & 255) == 0) {
+        struct timespec tn;
+        clock_gettime(CLOCK_MONOTONIC, &tn);
+        if ((double)(tn.tv_sec - g_t0.tv_sec) + 1e-9 * (double)(tn.tv_nsec - g_t0.tv_nsec) >= g_max_seconds)
+            g_max_pops = g_pops;
     }
     if (g_max_pops > 0 && g_pops >= g_max_pops) {
         struct timespec t1;

@@ -39,6 +39,7 @@ def measure(eng, seconds, fast):
         out["fp64_pair_evals_per_s"] = out["pair_evals_per_s"]
         out["pair_evals_per_s"] += out["fp32_screens_per_s"]
         out["events_per_round"] = (s1["pops"] - s0["pops"]) / max(f1["batches"] - f0["batches"], 1)
+        out["helpers"] = f1["helpers"]
     eng.close()
     return out
 
