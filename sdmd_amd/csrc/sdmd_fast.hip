@@ -269,6 +269,7 @@ struct FWave {
     unsigned int con_ez[FCON];
     float con_dmin[FCON], con_dmax[FCON];
     float con_rad[FCON][FCONSTEP];
+    int hbl_n;                   // candidates in hbl_j / hbl_k from the screen, -1: none listed
 };
 
 __device__ __forceinline__ int frow_bits(const FWave &wm, const Sys &S, const FAtom &A, int j) {
@@ -904,6 +905,33 @@ __device__ __forceinline__ void collision_fast(FSys &F, const ScrTab &T, FWave &
             r0[q] = G(F.rec)[2 * at];
             r1[q] = G(F.rec)[2 * at + 1];
         }
+        if (A.role == 'A' || A.role == 'D' || A.role == 'S') {
+            // a free H-bond atom scans the same neighbourhood for HBTime:
+            // list the candidates inside the scan radius (FP32 with the
+            // screen's band, so the list holds every candidate the FP64
+            // test keeps), in candidate order, for that scan
+            int off = 0;
+#pragma unroll
+            for (int q = 0; q < FQ; q++) {
+                const int k = lane + 64 * q;
+                const int j = __float_as_int(r1[q].w) >> 8;
+                bool nr = false;
+                if (k < n && j >= 1 && j <= S.N && j != A.i) {
+                    const float dtj = tA - r0[q].w;
+                    const float rx = s0.x - (r0[q].x + r1[q].x * dtj) - c_OX[cq[q]] * F.L32[0];
+                    const float ry = s0.y - (r0[q].y + r1[q].y * dtj) - c_OY[cq[q]] * F.L32[1];
+                    const float rz = s0.z - (r0[q].z + r1[q].z * dtj) - c_OZ[cq[q]] * F.L32[2];
+                    nr = rx * rx + ry * ry + rz * rz <= calib + cband;
+                }
+                const unsigned long long m = __ballot(nr);
+                const int r = off + __builtin_amdgcn_mbcnt_hi((unsigned int)(m >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((unsigned int)m, 0));
+                if (nr && r < 128) { wm.hbl_j[r] = j; wm.hbl_k[r] = k; }
+                off += __popcll(m);
+            }
+            if (lane == 0) wm.hbl_n = off <= 128 ? off : -1;
+            __builtin_amdgcn_wave_barrier();
+        }
         CandS o[FQ];
         float thi = FINF;
 #pragma unroll
@@ -1048,6 +1076,36 @@ __device__ __forceinline__ bool hbfree_pass(const FSys &F, const FAtom &A, int j
     return pairok && G(S.hbbond)[j] == 0 && ((A.protein == pj && sep >= 4) || A.protein != pj);
 }
 
+// the whole candidate with j's bookkeeping and state loaded together (for
+// candidates already known to lie near A)
+__device__ __forceinline__ double hbfree_all(const FSys &F, const FAtom &A, int j, double calib, double &pot, bool &reached,
+                                             int *err) {
+    const Sys &S = F.s;
+    reached = false;
+    pot = 0;
+    const signed char rj = G(S.role)[j];
+    const int tj = G(S.type)[j], aj = G(S.aanum)[j], pj = G(S.protein)[j], hj = G(S.hbbond)[j];
+    Kin k;
+    kin_local(F, A, j, k);
+    const signed char ri = A.role;
+    const bool pairok = (ri == 'A' && rj == 'D') || (ri == 'D' && rj == 'A') || (ri == 'S' && tj == 2) ||
+                        (rj == 'S' && A.type == 2) || (ri == 'S' && rj == 'S');
+    int sep = A.aanum - aj;
+    if (sep < 0) sep = -sep;
+    if (!(pairok && hj == 0 && ((A.protein == pj && sep >= 4) || A.protein != pj))) return __builtin_inf();
+    if (!(k.rr <= calib) || check_pair_match(S, A.i, j, err) < 0) return __builtin_inf();
+    FP f;
+    const int cs = find_pair(S, A.i, j, K_HB, k.b, k.rr, f, 0, 0, err);
+    reached = true;
+    if (k.b < 0 && cs == 1 && calc_disc(k, f.lo) > 0) {
+        pot = f.lopot;
+        double t = calc_time(k, -1, f.lo);
+        if (t < 0) t = INFINIT;
+        return (t > 0) ? t : __builtin_inf();
+    }
+    return __builtin_inf();
+}
+
 __device__ double hbfree64(const FSys &F, const FAtom &A, int j, double calib, double &pot, bool &reached, int *err) {
     const Sys &S = F.s;
     reached = false;
@@ -1111,6 +1169,8 @@ __device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm
     }
     PROF_MARK(S, FPH_BOND, pt);
     // CollisionTime (screened), against no running best yet
+    if (lane == 0) wm.hbl_n = -1;
+    __builtin_amdgcn_wave_barrier();
     Best cw = init;
     cw.t = __builtin_inf();
     collision_fast<FQ>(F, T, wm, A, cw, nscreen, nconf, err);
@@ -1148,26 +1208,60 @@ __device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm
     const int hbit = G(S.hbitype)[i];
     __builtin_amdgcn_wave_barrier();
     if (lane == 0) G(S.hbitype)[i] = SDMD_NOEVENT;
+    // The bonded H-bond partner's time (lane 32) and HBNeighborTime's pair
+    // times over neighborPartner[1..4] (lane q - 1 for slot q) read state
+    // only, so they are solved at once here and applied below in the
+    // reference's order (HBTime first, then the slots in order)
+    const int nq = min(A.nbstat, SDMD_NBSLOTS - 1);
+    const bool hbb = hbit != SDMD_HBSKIP && A.hbbond > 0;
+    double pt_t = INFINIT, pt_pot = 0;
+    int pt_sub = 0;
+    if (lane < nq) {
+        // HBNeighborTime slot q, TimePrediction.c:486-579
+        const int j = A.nbp[lane + 1];
+        Kin k;
+        kin_local(F, A, j, k);
+        int host, nb, partner;
+        const bool ads = (A.role == 'A' || A.role == 'D' || A.role == 'S');
+        if (ads && A.hbbond && G(S.hbnb)[A.hbbond] == j) { host = i; nb = j; partner = A.hbbond; }
+        else { host = j; nb = i; partner = G(S.hbbond)[j]; }
+        FP f;
+        const int cs = find_pair(S, nb, host, K_NEIGHBOR, k.b, k.rr, f, partner, 0, err);
+        int s;
+        double d2;
+        if (k.b < 0 && calc_disc(k, f.lo) > 0) {
+            s = -1; d2 = f.lo; pt_pot = f.lopot;
+            pt_sub = (cs == 0) ? SDMD_CORECOLLI : SDMD_TBDCAPTR;
+        } else {
+            s = 1; d2 = f.hi; pt_pot = f.hipot;
+            pt_sub = (cs == 1) ? SDMD_WELLBOUNC : SDMD_TBDESCAP;
+        }
+        pt_t = calc_time(k, s, d2);
+        if (pt_t < 0) pt_t = INFINIT;
+    } else if (lane == 32 && hbb) {
+        // bonded HB partner, TimePrediction.c:330-383
+        const int j = A.hbbond;
+        Kin k;
+        kin_local(F, A, j, k);
+        FP f;
+        const int cs = find_pair(S, i, j, K_HB, k.b, k.rr, f, 0, 0, err);
+        int s;
+        double d2;
+        if (k.b < 0 && calc_disc(k, f.lo) > 0) {
+            s = -1; d2 = f.lo; pt_pot = f.lopot;
+            pt_sub = (cs == 0) ? SDMD_CORECOLLI : SDMD_TBDCAPTR;
+        } else {
+            s = 1; d2 = f.hi; pt_pot = f.hipot;
+            pt_sub = (cs == 1) ? SDMD_TBDHBBRIK : SDMD_TBDESCAP;
+        }
+        pt_t = calc_time(k, s, d2);
+        if (pt_t < 0) pt_t = INFINIT;
+    }
     if (hbit != SDMD_HBSKIP) {
         if (A.hbbond > 0) {
-            // bonded HB partner, TimePrediction.c:330-383
-            const int j = A.hbbond;
-            Kin k;
-            kin_local(F, A, j, k);
-            FP f;
-            const int cs = find_pair(S, i, j, K_HB, k.b, k.rr, f, 0, 0, err);
-            int s, sub;
-            double d2, pot;
-            if (k.b < 0 && calc_disc(k, f.lo) > 0) {
-                s = -1; d2 = f.lo; pot = f.lopot;
-                sub = (cs == 0) ? SDMD_CORECOLLI : SDMD_TBDCAPTR;
-            } else {
-                s = 1; d2 = f.hi; pot = f.hipot;
-                sub = (cs == 1) ? SDMD_TBDHBBRIK : SDMD_TBDESCAP;
-            }
-            double tt = calc_time(k, s, d2);
-            if (tt < 0) tt = INFINIT;
-            take(best, S, tt, pot, sub, SDMD_HB, j);
+            const double tt = __shfl(pt_t, 32, 64), pot = __shfl(pt_pot, 32, 64);
+            const int sub = __shfl(pt_sub, 32, 64);
+            take(best, S, tt, pot, sub, SDMD_HB, A.hbbond);
             nfp64 += (lane == 0) ? 1 : 0;
         } else if (A.role == 'A' || A.role == 'D' || A.role == 'S') {
             // free HB atom: scan the neighbourhood in FP64, 128 candidates at
@@ -1180,7 +1274,18 @@ __device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm
             mine.t = __builtin_inf();
             int mink = 0x7fffffff;
             int my = 0;
-            for (int base = 0; base < n; base += 128) {
+            const int nl = wm.hbl_n;
+            // the candidates the screen listed inside the scan radius, in
+            // candidate order; otherwise every candidate, filtered below
+            for (int r = lane; r < nl; r += 64) {
+                const int j = wm.hbl_j[r], k = wm.hbl_k[r];
+                double pot;
+                bool reached;
+                const double tt = hbfree_all(F, A, j, calib, pot, reached, err);
+                my += reached;
+                if (tt < mine.t) { mine.t = tt; mine.pot = pot; mine.partner = j; mink = k; }
+            }
+            for (int base = 0; nl < 0 && base < n; base += 128) {
                 int jj[2];
                 bool ok[2];
 #pragma unroll
@@ -1230,34 +1335,18 @@ __device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm
             }
         }
     }
-    // HBNeighborTime, TimePrediction.c:486-579
+    // HBNeighborTime, TimePrediction.c:486-579: the slots' times (solved
+    // above) through JobAssign in slot order; neighborPartnerNum is the
+    // winning slot, else the last one
     {
         int flag = 0;
-        for (int q = 1; q <= A.nbstat && q < SDMD_NBSLOTS; q++) {
-            const int j = A.nbp[q];
+        for (int q = 1; q <= nq; q++) {
+            const double tt = __shfl(pt_t, q - 1, 64), pot = __shfl(pt_pot, q - 1, 64);
+            const int sub = __shfl(pt_sub, q - 1, 64);
             if (flag == 0) A.nbnum = q;
-            Kin k;
-            kin_local(F, A, j, k);
-            int host, nb, partner;
-            const bool ads = (A.role == 'A' || A.role == 'D' || A.role == 'S');
-            if (ads && A.hbbond && G(S.hbnb)[A.hbbond] == j) { host = i; nb = j; partner = A.hbbond; }
-            else { host = j; nb = i; partner = G(S.hbbond)[j]; }
-            FP f;
-            const int cs = find_pair(S, nb, host, K_NEIGHBOR, k.b, k.rr, f, partner, 0, err);
-            int s, sub;
-            double d2, pot;
-            if (k.b < 0 && calc_disc(k, f.lo) > 0) {
-                s = -1; d2 = f.lo; pot = f.lopot;
-                sub = (cs == 0) ? SDMD_CORECOLLI : SDMD_TBDCAPTR;
-            } else {
-                s = 1; d2 = f.hi; pot = f.hipot;
-                sub = (cs == 1) ? SDMD_WELLBOUNC : SDMD_TBDESCAP;
-            }
-            double tt = calc_time(k, s, d2);
-            if (tt < 0) tt = INFINIT;
             nfp64 += (lane == 0) ? 1 : 0;
             if (tt > 0 && best.t > tt) {
-                take(best, S, tt, pot, sub, SDMD_HBNE, j);
+                take(best, S, tt, pot, sub, SDMD_HBNE, A.nbp[q]);
                 A.nbnum = q;
                 flag = 1;
             }

@@ -75,6 +75,24 @@ def test_fast_bulk_prediction_matches_oracle_multichain(require_gpu, mid_nve):
     _bulk_check(mid_nve)
 
 
+def test_fast_bulk_prediction_with_hbonds_matches_oracle(require_gpu):
+    """A state with H bonds formed (the oracle's state after 1.5*10^5 pops
+    of the reference's ala10.gro run, thermostat then switched off so no
+    random draw enters): bonded H-bond partners (HBTime's bonded branch)
+    and H-bond neighbour slots (HBNeighborTime) enter the bulk prediction."""
+    from sdmd_amd.engine import System
+
+    s, _ = golden_system("ala10helix")
+    o = oracle_lib.Oracle(s.arrays)
+    o.bulk()
+    o.run(150000, trace=False)
+    arr = o.state()
+    o.close()
+    arr.scalars["thermostat"] = 0
+    assert (arr.arr["hb_bond"][1:] > 0).sum() >= 4 and (arr.arr["nb_status"][1:] > 0).sum() >= 8
+    _bulk_check(System(None, arr))
+
+
 def test_fast_screen_agrees_with_full_fp64_scan(require_gpu, mid_nvt):
     from sdmd_amd.engine import GpuEngine
 
