@@ -75,6 +75,27 @@ def test_fast_bulk_prediction_matches_oracle_multichain(require_gpu, mid_nve):
     _bulk_check(mid_nve)
 
 
+def test_fast_screen_boundary_cases_match_oracle(require_gpu, tmp_path):
+    """Pairs placed on the screen's decision boundaries (separations at
+    every shell radius of the pair times 1 +- eps, eps down to 1e-12;
+    approaching, receding, b = 0, grazing at impact parameters on the
+    radii; just beyond the last step; tests/screen_cases.py): the fast
+    bulk prediction equals the oracle's for every atom, and a run with
+    every prediction re-solved by a full FP64 scan agrees throughout."""
+    import screen_cases
+    from sdmd_amd.engine import GpuEngine
+
+    sysm, _, _ = screen_cases.boundary_system(tmp_path)
+    _bulk_check(sysm)
+    eng = GpuEngine([sysm], block=512, mode="fast", validate=True)
+    eng.prepare()
+    eng.run(20000)
+    fs, st = eng.fast_stats(0), eng.stats(0)
+    eng.close()
+    assert st["error"] == 0 and fs["causal"] == 0
+    assert fs["validated"] > 20000 and fs["agree"] == fs["validated"], fs
+
+
 def test_fast_bulk_prediction_with_hbonds_matches_oracle(require_gpu):
     """A state with H bonds formed (the oracle's state after 1.5*10^5 pops
     of the reference's ala10.gro run, thermostat then switched off so no

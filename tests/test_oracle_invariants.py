@@ -260,3 +260,19 @@ def test_wall_events_reflect_or_cross_radially():
         kinds.add(("bounce" if bounce else "cross", int(tr["sub"][k])))
     # this run has TBDCaptr and TBDEscap events, both crossing steps
     assert {k[1] for k in kinds} >= {4, 5}
+
+
+def test_screen_boundary_cases_are_built_as_designed(tmp_path):
+    """The adversarial pair geometries (tests/screen_cases.py): every
+    experiment's chains keep their bonds (no bond event), and the designed
+    pair is the predicted contact of one of its atoms in most experiments,
+    so the near-boundary decisions are the ones that choose the winners."""
+    import screen_cases
+
+    sysm, exps, pairs = screen_cases.boundary_system(tmp_path)
+    o = oracle_lib.Oracle(sysm.arrays)
+    assert o.bulk() == 0
+    st = o.state().arr
+    assert np.all(st["ev_type"][1:] != 2)                    # SDMD_BOND
+    won = [st["ev_partner"][ia] == ib or st["ev_partner"][ib] == ia for ia, ib in pairs]
+    assert len(exps) > 700 and np.mean(won) > 0.4
