@@ -270,6 +270,7 @@ struct FWave {
     float con_dmin[FCON], con_dmax[FCON];
     float con_rad[FCON][FCONSTEP];
     int hbl_n;                   // candidates in hbl_j / hbl_k from the screen, -1: none listed
+    WStats st;                   // the wavefront's statistics (wave-uniform updates)
 };
 
 __device__ __forceinline__ int frow_bits(const FWave &wm, const Sys &S, const FAtom &A, int j) {
@@ -1165,7 +1166,7 @@ __device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm
         int win;
         wave_best(mine, q, best, out, win);
         best = out;
-        nfp64 += (lane == 0) ? nb : 0;
+        nfp64 += nb;
     }
     PROF_MARK(S, FPH_BOND, pt);
     // CollisionTime (screened), against no running best yet
@@ -1262,7 +1263,7 @@ __device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm
             const double tt = __shfl(pt_t, 32, 64), pot = __shfl(pt_pot, 32, 64);
             const int sub = __shfl(pt_sub, 32, 64);
             take(best, S, tt, pot, sub, SDMD_HB, A.hbbond);
-            nfp64 += (lane == 0) ? 1 : 0;
+            nfp64 += 1;
         } else if (A.role == 'A' || A.role == 'D' || A.role == 'S') {
             // free HB atom: scan the neighbourhood in FP64, 128 candidates at
             // a time: the conditions on j's bookkeeping for two candidates
@@ -1344,7 +1345,7 @@ __device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm
             const double tt = __shfl(pt_t, q - 1, 64), pot = __shfl(pt_pot, q - 1, 64);
             const int sub = __shfl(pt_sub, q - 1, 64);
             if (flag == 0) A.nbnum = q;
-            nfp64 += (lane == 0) ? 1 : 0;
+            nfp64 += 1;
             if (tt > 0 && best.t > tt) {
                 take(best, S, tt, pot, sub, SDMD_HBNE, A.nbp[q]);
                 A.nbnum = q;
@@ -1705,7 +1706,9 @@ __global__ __launch_bounds__(64 * W) void k_fast_prepare(FSys *systems) {
     __syncthreads();
     for (int a = tid + 1; a <= S.N; a += 64 * W) put_record(F, a);
     __syncthreads();
-    WStats ws = {0, 0, 0, 0};
+    WStats &ws = sh.w[wave].st;             // in LDS: no registers held across the loop
+    ws.screen = ws.conf = ws.fp64 = ws.cand = 0;
+    __builtin_amdgcn_wave_barrier();
     int err = 0;
     for (int i = wave + 1; i <= S.N; i += W) {
         predict_fast<4>(F, tab, sh.w[wave], i, G(F.t0)[i], ws, &err);
@@ -1735,7 +1738,9 @@ __global__ __launch_bounds__(64 * W) void k_fast_prepare(FSys *systems) {
 template <int W, int FQ>
 __device__ __forceinline__ void helper_loop(FSys &F, const ScrTab &tab, FShared &sh, FJobs *J, FHelp *hp, int xcc) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    WStats ws = {0, 0, 0, 0};
+    WStats &ws = sh.w[wave].st;             // in LDS: no registers held across the loop
+    ws.screen = ws.conf = ws.fp64 = ws.cand = 0;
+    __builtin_amdgcn_wave_barrier();
     for (;;) {
         int q = -1, quit = 0;
         if (lane == 0) {
@@ -1828,7 +1833,9 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, FJobs *jobs,
     __syncthreads();
     const long long t_start = (long long)wall_clock64();
     const long long pops0 = S.pops, frame0 = S.frame;     // npops < 0: -npops executed events
-    WStats ws = {0, 0, 0, 0};
+    WStats &ws = sh.w[wave].st;             // in LDS: no registers held across the loop
+    ws.screen = ws.conf = ws.fp64 = ws.cand = 0;
+    __builtin_amdgcn_wave_barrier();
     const int cap = F.max_batch > 0 ? min(F.max_batch, FSEL) : FSEL;
     unsigned long long pt = S.prof ? wall_clock64() : 0;
     while (true) {
