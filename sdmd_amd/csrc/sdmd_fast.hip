@@ -859,7 +859,10 @@ __device__ __forceinline__ void screen_at(const FSys &F, const ScrTab &T, const 
 // the marked candidates and of those whose interval reaches the smallest
 // upper bound; the FP64 minimum (first in scan order on ties) competes with
 // the running best under JobAssign's strict '<'
-template <int FQ>
+// VAL: validation mode (every live candidate also solved in FP64 and the
+// full scan's winner compared), a separate instantiation so that the
+// screened path carries none of its state
+template <int FQ, bool VAL>
 __device__ __forceinline__ void collision_fast(FSys &F, const ScrTab &T, FWave &wm, const FAtom &A, Best &best, long long &nscreen,
                                long long &nconf, int *err) {
     const Sys &S = F.s;
@@ -889,7 +892,9 @@ __device__ __forceinline__ void collision_fast(FSys &F, const ScrTab &T, FWave &
             my_conf++;
             if (t < mine.t) { mine.t = t; mine.pot = pot; mine.sub = sub; mine.partner = o.j; mink = k; myt32 = o.t; }
         }
-        if (t < full.t) { full.t = t; full.pot = pot; full.sub = sub; full.partner = o.j; fullk = k; }
+        if constexpr (VAL) {
+            if (t < full.t) { full.t = t; full.pot = pot; full.sub = sub; full.partner = o.j; fullk = k; }
+        }
     };
     if (n <= 64 * FQ) {
         // one pass: all record loads issued together, screens kept in registers
@@ -952,7 +957,7 @@ __device__ __forceinline__ void collision_fast(FSys &F, const ScrTab &T, FWave &
         for (int q = 0; q < FQ; q++) {
             const bool need = o[q].live && (o[q].flag || (o[q].t < FINF && o[q].t - o[q].e <= thi));
             needm |= (need ? 1u : 0u) << q;
-            valm |= ((o[q].live && F.validate) ? 1u : 0u) << q;
+            valm |= ((o[q].live && VAL) ? 1u : 0u) << q;
         }
         unsigned todo = needm | valm;
         while (todo) {
@@ -987,7 +992,7 @@ __device__ __forceinline__ void collision_fast(FSys &F, const ScrTab &T, FWave &
             screen_at(F, T, wm, A, c, q0, q1, s0, s1, tA, calib, cband, o, dummy);
             if (!o.live) continue;
             const bool need = o.flag || (o.t < FINF && o.t - o.e <= thi);
-            if (need || F.validate) confirm(k, o, need);
+            if (need || VAL) confirm(k, o, need);
         }
     }
     PROF_MARK(S, FPH_CONFIRM, pt);
@@ -1011,7 +1016,7 @@ __device__ __forceinline__ void collision_fast(FSys &F, const ScrTab &T, FWave &
         if (m && lane == 0 && win.t > 0 && w32 < FINF) cnt_max(F.max_screen_err, fabs((double)w32 - win.t) / win.t);
         if (!m) win.t = __builtin_inf();
     }
-    if (F.validate) {
+    if constexpr (VAL) {
         double tf = full.t;
         int kf = fullk;
         wave_argmin(tf, kf);
@@ -1174,7 +1179,8 @@ __device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm
     __builtin_amdgcn_wave_barrier();
     Best cw = init;
     cw.t = __builtin_inf();
-    collision_fast<FQ>(F, T, wm, A, cw, nscreen, nconf, err);
+    if (F.validate) collision_fast<FQ, true>(F, T, wm, A, cw, nscreen, nconf, err);
+    else collision_fast<FQ, false>(F, T, wm, A, cw, nscreen, nconf, err);
     if (S.prof) pt = wall_clock64();
     // JobAssign order: bonds first, then the collision winner if strictly earlier
     if (cw.partner > 0 && cw.t > 0 && best.t > cw.t) best = cw;
@@ -1621,6 +1627,7 @@ __global__ void k_fast_jobs_reset(FJobs *jobs, int nsys) {
 
 // ---------------------------------------------------------------- shared memory
 struct FShared {
+    long long run_clock_end, run_count_end;   // the launch's bounds (thread 0)
     double b1[FMAXBLK], b2[FMAXBLK], b3[FMAXBLK];
     int ba[FMAXBLK], bb[FMAXBLK];
     double red_t[FMAXW], red_t1[FMAXW], red_t3[FMAXW];
@@ -1831,8 +1838,10 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, FJobs *jobs,
     if (tid == 0) { sh.err = S.error; sh.stop = 0; sh.drain = 0; }
     if (tid < FMAXBLK / 32) sh.dmask[tid] = 0;
     __syncthreads();
-    const long long t_start = (long long)wall_clock64();
-    const long long pops0 = S.pops, frame0 = S.frame;     // npops < 0: -npops executed events
+    if (tid == 0) {                       // npops < 0: -npops executed events
+        sh.run_clock_end = (long long)wall_clock64() + budget;
+        sh.run_count_end = npops < 0 ? S.frame - npops : S.pops + npops;
+    }
     WStats &ws = sh.w[wave].st;             // in LDS: no registers held across the loop
     ws.screen = ws.conf = ws.fp64 = ws.cand = 0;
     __builtin_amdgcn_wave_barrier();
@@ -1872,8 +1881,8 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, FJobs *jobs,
             // still pending up to the latest executed time, so that every
             // atom's local time is at most that time and every pending event
             // is later -- the state is then a snapshot at one time.
-            const bool spent = npops < 0 ? (S.frame - frame0 >= -npops) : (S.pops - pops0 >= npops);
-            if (!sh.drain && (spent || ((long long)wall_clock64() - t_start) > budget)) {
+            const bool spent = (npops < 0 ? S.frame : S.pops) >= sh.run_count_end;
+            if (!sh.drain && (spent || (long long)wall_clock64() > sh.run_clock_end)) {
                 sh.drain = 1;
                 if (F.tlast < F.t_stop) F.t_stop = F.tlast;
                 sh.drain_end = S.pops + FDRAIN;
