@@ -1589,9 +1589,12 @@ __device__ __forceinline__ unsigned long long job_load(unsigned long long *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ int int_load(int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima): this wave's
-// vector-memory operations have completed
-__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+// s_waitcnt vmcnt(0): this wave's vector-memory operations (its stores
+// included) have completed in the L2.  Written as an asm statement with a
+// memory clobber: the compiler neither moves memory operations across it nor
+// drops it (the s_waitcnt builtin was found removed from the event loop by
+// the wait-count pass, and it does not order memory operations)
+__device__ __forceinline__ void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 // agent-scope release (L2 write-back), waited for before the caller's next
 // store, which publishes what came before
 __device__ __forceinline__ void release_agent() {
