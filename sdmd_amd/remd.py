@@ -183,6 +183,7 @@ def main():
         "attempts": [int(a) for a in r.counts[0:2 * (r.n - 1):2]],
         "wall_events": int(sum(s["per_type"][5] for s in st)),
         "errors": int(sum(s["error"] != 0 for s in st)),
+        "causal_reads": int(sum(r.eng.fast_stats(k)["causal"] for k in range(r.n))),
         "setup_s": setup_s,
         "temperature_index_of_replica": [int(x) for x in nums],
     }
