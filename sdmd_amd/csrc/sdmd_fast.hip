@@ -1447,10 +1447,147 @@ __device__ void thermostat_fast(FSys &F, int i) {
 
 // execute one popped event on lane 0 (DoEvent, Events.c:25-114, with the
 // counter increments of SGThread.c:100-104); returns the renew count
-__device__ __forceinline__ int exec_fast(FSys &F, int a, double t, int *renew, int *err) {
+// mirror record of atom a from values the caller holds (as put_record)
+__device__ __forceinline__ void put_record_v(FSys &F, int a, int cx, int cy, int cz, int slot, int type, double x,
+                                             double y, double z, double vx, double vy, double vz, double t0) {
+    const Sys &S = F.s;
+    const size_t at = (size_t)cell_id(S, cx, cy, cz) * F.cap + slot;
+    fvec4 r0, r1;
+    r0.x = (float)(x - cx * S.cellsize[1]);
+    r0.y = (float)(y - cy * S.cellsize[2]);
+    r0.z = (float)(z - cz * S.cellsize[3]);
+    r0.w = (float)(t0 - F.Te);
+    r1.x = (float)vx;
+    r1.y = (float)vy;
+    r1.z = (float)vz;
+    r1.w = __int_as_float((a << 8) | (type & 255));
+    G(F.rec)[2 * at] = r0;
+    G(F.rec)[2 * at + 1] = r1;
+}
+
+// one atom's state for a pair event, loaded at once
+struct PairAtom {
+    double x, y, z, vx, vy, vz, t0, m;
+    long long c;
+    int cx, cy, cz, slot, type;
+};
+
+__device__ __forceinline__ void pair_load(const FSys &F, int j, PairAtom &q) {
+    const Sys &S = F.s;
+    q.x = G(S.x)[j]; q.y = G(S.y)[j]; q.z = G(S.z)[j];
+    q.vx = G(S.vx)[j]; q.vy = G(S.vy)[j]; q.vz = G(S.vz)[j];
+    q.t0 = G(F.t0)[j]; q.m = G(S.mass)[j]; q.c = G(S.evc)[j];
+    q.cx = G(S.cx)[j]; q.cy = G(S.cy)[j]; q.cz = G(S.cz)[j];
+    q.slot = G(F.slot)[j]; q.type = G(S.type)[j];
+}
+
+// advance to t (as advance()) in registers, stored
+__device__ __forceinline__ void pair_advance(FSys &F, int j, PairAtom &q, double t) {
     Sys &S = F.s;
+    const double dt = t - q.t0;
+    q.x = q.x + q.vx * dt;
+    q.y = q.y + q.vy * dt;
+    q.z = q.z + q.vz * dt;
+    q.t0 = t;
+    G(S.x)[j] = q.x; G(S.y)[j] = q.y; G(S.z)[j] = q.z;
+    G(F.t0)[j] = t;
+}
+
+// A collision, bond or H-bond-neighbour event of atoms a and p (DoEvent ->
+// InteractionEvent without its H-bond branch, Events.c:770-1025), with
+// both atoms' state loaded in two round trips (a's, then p's) and the same
+// arithmetic as exec_fast's general path (advance, PosLocal, image_near,
+// interaction_event): both mirror records are written here.  Returns the
+// renewed atoms (1 when the event is stale).
+__device__ __forceinline__ int exec_pair(FSys &F, int a, int p, int ty, double t, const PairAtom &a0, long long pca,
+                                         double pot0, int sub0, int *renew, int *err) {
+    Sys &S = F.s;
+    PairAtom A = a0, P;
+    pair_load(F, p, P);
+    pair_advance(F, a, A, t);
+    renew[0] = a;
+    if (pca < P.c) {                                           // HazardCheck, SGThread.c:215-235
+        cnt_add(S.recal, 1);
+        put_record_v(F, a, A.cx, A.cy, A.cz, A.slot, A.type, A.x, A.y, A.z, A.vx, A.vy, A.vz, A.t0);
+        return 1;
+    }
+    if (P.t0 > t) {
+        cnt_add(F.causal, 1);
+        *err = FERR_CAUSAL;
+    }
+    pair_advance(F, p, P, t);
+    // interaction_event: positions at t through the local-time view
+    // (x + v (t - t0) with t0 = t) and the cell-based nearest image
+    const double z0 = t - t;
+    const double pi0 = A.x + A.vx * z0, pi1 = A.y + A.vy * z0, pi2 = A.z + A.vz * z0;
+    double pj[3] = {P.x + P.vx * z0, P.y + P.vy * z0, P.z + P.vz * z0};
+    const int ci[3] = {A.cx, A.cy, A.cz}, cj[3] = {P.cx, P.cy, P.cz};
+    for (int n = 0; n < 3; n++) {
+        if (ci[n] != cj[n]) {
+            if (ci[n] == 0 && cj[n] == S.cellnum[n + 1] - 1) pj[n] -= S.box[n + 1];
+            else if (ci[n] == S.cellnum[n + 1] - 1 && cj[n] == 0) pj[n] += S.box[n + 1];
+        }
+    }
+    const double r1 = pi0 - pj[0], r2 = pi1 - pj[1], r3 = pi2 - pj[2];
+    const double v1 = A.vx - P.vx, v2 = A.vy - P.vy, v3 = A.vz - P.vz;
+    const double d2 = r1 * r1 + r2 * r2 + r3 * r3;
+    const double b = r1 * v1 + r2 * v2 + r3 * v3;
+    const double mu = A.m * P.m / (A.m + P.m);
+    double pot = pot0;
+    int s = 0;
+    bool ok = true;
+    switch (sub0) {
+    case SDMD_CORECOLLI: s = 1; pot = 0; break;
+    case SDMD_WELLBOUNC: s = (b < 0) ? 1 : -1; pot = 0; break;
+    case SDMD_WELLCAPTR: s = -1; break;
+    case SDMD_WELLESCAP: s = (b < 0) ? -1 : 1; break;
+    case SDMD_TBDCAPTR:
+        if (b * b - 2 * d2 * pot / mu <= 0) { pot = 0; s = 1; } else s = -1;
+        break;
+    case SDMD_TBDESCAP:
+        if (b * b - 2 * d2 * pot / mu <= 0) { pot = 0; s = (b < 0) ? 1 : -1; }
+        else s = (b < 0) ? -1 : 1;
+        break;
+    default: *err = ERR_SUBTYPE; ok = false; break;
+    }
+    if (ok) {
+        const double phi = (-1 * b + s * sqrt((b * b - 2 * d2 * pot / mu))) / ((A.m + P.m) * d2);
+        if (isnan(phi)) {
+            *err = ERR_PHI_NAN;
+        } else {
+            const double fi = P.m * phi, fj = A.m * phi;
+            A.vx = A.vx + fi * r1; A.vy = A.vy + fi * r2; A.vz = A.vz + fi * r3;
+            P.vx = P.vx - fj * r1; P.vy = P.vy - fj * r2; P.vz = P.vz - fj * r3;
+            G(S.vx)[a] = A.vx; G(S.vy)[a] = A.vy; G(S.vz)[a] = A.vz;
+            G(S.vx)[p] = P.vx; G(S.vy)[p] = P.vy; G(S.vz)[p] = P.vz;
+        }
+    }
+    cnt_add(S.per_type[ty], 1);
+    cnt_add(S.frame, 1);
+    G(S.evc)[a] = A.c + 1;
+    G(S.evc)[p] = P.c + 1;
+    put_record_v(F, a, A.cx, A.cy, A.cz, A.slot, A.type, A.x, A.y, A.z, A.vx, A.vy, A.vz, A.t0);
+    put_record_v(F, p, P.cx, P.cy, P.cz, P.slot, P.type, P.x, P.y, P.z, P.vx, P.vy, P.vz, P.t0);
+    renew[1] = p;
+    return 2;
+}
+
+__device__ __forceinline__ int exec_fast(FSys &F, int a, double t, int *renew, int *err, bool *recs) {
+    Sys &S = F.s;
+    // the record and state of a, loaded together with its partner id
     const int p = G(S.evp)[a];
+    const int ty0 = G(S.evtype)[a];
+    const long long pca = G(S.evpc)[a];
+    const double pot0 = G(S.evpot)[a];
+    const int sub0 = G(S.evsub)[a];
+    PairAtom a0;
+    pair_load(F, a, a0);
     const int partner = (p > 0 && p <= S.N) ? p : 0;
+    *recs = false;
+    if (partner && partner != a && (ty0 == SDMD_COLI || ty0 == SDMD_BOND || ty0 == SDMD_HBNE)) {
+        *recs = true;
+        return exec_pair(F, a, partner, ty0, t, a0, pca, pot0, sub0, renew, err);
+    }
     advance(F, a, t);
     renew[0] = a;
     if (partner && G(S.evpc)[a] < G(S.evc)[partner]) {          // HazardCheck, SGThread.c:215-235
@@ -2053,9 +2190,12 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, FJobs *jobs,
                 const int p = G(S.evp)[a];
                 const signed char ra = G(S.role)[a], rb = G(S.role)[min(max(p, 0), S.N)];
                 int renew[2] = {0, 0};
-                const int nr = exec_fast(F, a, t, renew, &err);
-                put_record(F, renew[0]);
-                if (nr > 1) put_record(F, renew[1]);
+                bool recs;
+                const int nr = exec_fast(F, a, t, renew, &err, &recs);
+                if (!recs) {                               // the pair path writes its own records
+                    put_record(F, renew[0]);
+                    if (nr > 1) put_record(F, renew[1]);
+                }
                 cnt_max(F.tlast, t);
                 for (int r = 0; r < nr; r++) {
                     const signed char ro = r ? rb : ra;          // renew[1] is the partner p
