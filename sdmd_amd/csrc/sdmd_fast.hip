@@ -270,6 +270,7 @@ struct FWave {
     float con_dmin[FCON], con_dmax[FCON];
     float con_rad[FCON][FCONSTEP];
     int hbl_n;                   // candidates in hbl_j / hbl_k from the screen, -1: none listed
+    unsigned char conidx[128];   // constraint slot + 1 of the partner at row offset j - rowlo (0: none)
     WStats st;                   // the wavefront's statistics (wave-uniform updates)
 };
 
@@ -696,9 +697,13 @@ __device__ __forceinline__ void fstage(FSys &F, const ScrTab &T, FWave &wm, FAto
             A.mb[0] = mb0; A.mb[1] = mb1; A.mc[0] = mc0; A.mc[1] = mc1;
         }
     }
-    // ---- constraint shells (one lane each)
+    // ---- constraint shells (one lane each), and their slots by row offset
+    // when the row spans fewer than 128 ids
     {
         int ok = 1;
+        wm.conidx[lane] = 0;
+        wm.conidx[lane + 64] = 0;
+        if (own_con && spanok && cj >= rowlo && cj - rowlo < 128) wm.conidx[cj - rowlo] = (unsigned char)(lane + 1);
         if (own_con) {
             wm.con_j[lane] = cj;
             wm.con_dmin[lane] = (float)cdmin;
@@ -814,8 +819,13 @@ __device__ __forceinline__ void screen_at(const FSys &F, const ScrTab &T, const 
     Scr sc;
     if (con && A.constaged) {
         int q = -1;
-        for (int k = 0; k < A.ncon; k++)
-            if (wm.con_j[k] == j) { q = k; break; }
+        const int off = j - A.rowlo;
+        if (A.spanok && off >= 0 && off < 128) {
+            q = (int)wm.conidx[off] - 1;
+        } else {
+            for (int k = 0; k < A.ncon; k++)
+                if (wm.con_j[k] == j) { q = k; break; }
+        }
         if (q < 0) {
             sc.flag = true; sc.t = FINF; sc.e = 0;
         } else {
