@@ -477,13 +477,96 @@ static __device__ void cmap_toggle(Sys &S, int i, int j, int bit, int *err) {
 }
 
 
-static __device__ __forceinline__ void wave_argmin(double &t, int &k) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        double t2 = __shfl_xor(t, off, 64);
-        int k2 = __shfl_xor(k, off, 64);
-        if (t2 < t || (t2 == t && k2 < k)) { t = t2; k = k2; }
+// ---- cross-lane exchange in registers (no LDS round trip).  Step S of a
+// wavefront all-reduce gives each lane two values: its own and that of a
+// lane in the other half of its aligned group of 2^(S+1) lanes (in either
+// order).  Steps 0-3 are DPP moves inside rows of 16 lanes (quad
+// permutations, then the half-row and row mirrors, which pair every lane
+// with the other half only once each half holds one value: the steps run in
+// the order 0..5); steps 4 and 5 are gfx950's row-pair and half-wave swaps.
+template <int S>
+static __device__ __forceinline__ void xpair(unsigned v, unsigned &a, unsigned &b) {
+    if constexpr (S < 4) {
+        constexpr int ctrl = S == 0 ? 0xB1 : S == 1 ? 0x4E : S == 2 ? 0x141 : 0x140;
+        a = v;
+        b = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, ctrl, 0xF, 0xF, false);
+    } else if constexpr (S == 4) {
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        a = r[0]; b = r[1];
+    } else {
+        const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        a = r[0]; b = r[1];
     }
+}
+template <int S>
+static __device__ __forceinline__ void xpair64(unsigned long long v, unsigned long long &a, unsigned long long &b) {
+    unsigned al, bl, ah, bh;
+    xpair<S>((unsigned)v, al, bl);
+    xpair<S>((unsigned)(v >> 32), ah, bh);
+    a = ((unsigned long long)ah << 32) | al;
+    b = ((unsigned long long)bh << 32) | bl;
+}
+// the value of lane L (uniform L) in every lane, as a scalar
+static __device__ __forceinline__ int lane_get(int v, int L) { return __builtin_amdgcn_readlane(v, L); }
+static __device__ __forceinline__ long long lane_get(long long v, int L) {
+    const int lo = __builtin_amdgcn_readlane((int)v, L), hi = __builtin_amdgcn_readlane((int)(v >> 32), L);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+static __device__ __forceinline__ double lane_get(double v, int L) {
+    return __longlong_as_double(lane_get(__double_as_longlong(v), L));
+}
+static __device__ __forceinline__ float lane_get(float v, int L) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), L));
+}
+
+template <int S>
+static __device__ __forceinline__ void argmin_step(double &t, int &k) {
+    unsigned long long ta, tb;
+    unsigned ka, kb;
+    xpair64<S>((unsigned long long)__double_as_longlong(t), ta, tb);
+    xpair<S>((unsigned)k, ka, kb);
+    const double t1 = __longlong_as_double((long long)ta), t2 = __longlong_as_double((long long)tb);
+    const int k1 = (int)ka, k2 = (int)kb;
+    const bool two = t2 < t1 || (t2 == t1 && k2 < k1);
+    t = two ? t2 : t1;
+    k = two ? k2 : k1;
+}
+// the wave's smallest (t, k), smallest k on equal t, in every lane
+static __device__ __forceinline__ void wave_argmin(double &t, int &k) {
+    argmin_step<0>(t, k); argmin_step<1>(t, k); argmin_step<2>(t, k);
+    argmin_step<3>(t, k); argmin_step<4>(t, k); argmin_step<5>(t, k);
+    t = lane_get(t, 0);
+    k = lane_get(k, 0);
+}
+template <int S>
+static __device__ __forceinline__ void dmin_step(double &t) {
+    unsigned long long a, b;
+    xpair64<S>((unsigned long long)__double_as_longlong(t), a, b);
+    t = fmin(__longlong_as_double((long long)a), __longlong_as_double((long long)b));
+}
+static __device__ __forceinline__ double wave_min_d(double t) {
+    dmin_step<0>(t); dmin_step<1>(t); dmin_step<2>(t); dmin_step<3>(t); dmin_step<4>(t); dmin_step<5>(t);
+    return lane_get(t, 0);
+}
+template <int S>
+static __device__ __forceinline__ void or_step(unsigned long long &m) {
+    unsigned long long a, b;
+    xpair64<S>(m, a, b);
+    m = a | b;
+}
+static __device__ __forceinline__ unsigned long long wave_or64(unsigned long long m) {
+    or_step<0>(m); or_step<1>(m); or_step<2>(m); or_step<3>(m); or_step<4>(m); or_step<5>(m);
+    return (unsigned long long)lane_get((long long)m, 0);
+}
+// inclusive prefix sum over lanes 0..31 (DPP row shifts, then row 0's total
+// into row 1; lanes 32..63 hold their own rows' sums the same way)
+static __device__ __forceinline__ int scan32_incl(int x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);     // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);     // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);     // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);     // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);     // row_bcast:15 into rows 1, 3
+    return x;
 }
 
 static __device__ int check_pair_match(const Sys &S, int i, int j, int *err) {   // CheckPairMatch 1119-1138

@@ -481,35 +481,36 @@ struct ScrTab {
 };
 
 // ---------------------------------------------------------------- wave helpers
-__device__ __forceinline__ void wave_argmin_f(float &t, int &k) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const float t2 = __shfl_xor(t, off, 64);
-        const int k2 = __shfl_xor(k, off, 64);
-        if (t2 < t || (t2 == t && k2 < k)) { t = t2; k = k2; }
-    }
+template <int S>
+__device__ __forceinline__ void fmin_step(float &t) {
+    unsigned a, b;
+    xpair<S>(__float_as_uint(t), a, b);
+    t = fminf(__uint_as_float(a), __uint_as_float(b));
 }
-
 __device__ __forceinline__ float wave_min_f(float t) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) t = fminf(t, __shfl_xor(t, off, 64));
-    return t;
+    fmin_step<0>(t); fmin_step<1>(t); fmin_step<2>(t); fmin_step<3>(t); fmin_step<4>(t); fmin_step<5>(t);
+    return lane_get(t, 0);
 }
 
+template <int S>
+__device__ __forceinline__ void isum_step(int &v) {
+    unsigned a, b;
+    xpair<S>((unsigned)v, a, b);
+    v = (int)(a + b);
+}
 __device__ __forceinline__ int wave_sum_i(int v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    isum_step<0>(v); isum_step<1>(v); isum_step<2>(v); isum_step<3>(v); isum_step<4>(v); isum_step<5>(v);
+    return lane_get(v, 0);
 }
 
-// broadcast a Best from lane src to the wave
+// broadcast a Best from lane src (uniform) to the wave
 __device__ __forceinline__ void bcast(Best &B, int src) {
-    B.t = __shfl(B.t, src, 64);
-    B.pot = __shfl(B.pot, src, 64);
-    B.pc = __shfl(B.pc, src, 64);
-    B.partner = __shfl(B.partner, src, 64);
-    B.type = __shfl(B.type, src, 64);
-    B.sub = __shfl(B.sub, src, 64);
+    B.t = lane_get(B.t, src);
+    B.pot = lane_get(B.pot, src);
+    B.pc = lane_get(B.pc, src);
+    B.partner = lane_get(B.partner, src);
+    B.type = lane_get(B.type, src);
+    B.sub = lane_get(B.sub, src);
 }
 
 // wave-ordered min over per-lane candidates with program-order index k:
@@ -597,7 +598,11 @@ __device__ __forceinline__ void fstage(FSys &F, const ScrTab &T, FWave &wm, FAto
         if (lane == 0) A.nbp[k] = v;
     }
     {
-        const int dj = __shfl(lo, 32 + (lane & 7), 64), db = __shfl(lo, 40 + (lane & 7), 64);
+        // lanes 0..7 take lanes 32..39 (half-wave swap) and 40..47 (then 8
+        // lanes down the row)
+        const auto sw = __builtin_amdgcn_permlane32_swap((unsigned)lo, (unsigned)lo, false, false);
+        const int dj = (int)sw[1];
+        const int db = __builtin_amdgcn_update_dpp(0, dj, 0x108, 0xF, 0xF, false);     // row_shl:8
         if (lane < SDMD_DYNSLOTS) { wm.dyn_j[lane] = dj; wm.dyn_b[lane] = db; }
     }
     // ---- level 2
@@ -610,7 +615,8 @@ __device__ __forceinline__ void fstage(FSys &F, const ScrTab &T, FWave &wm, FAto
     // bond q on lane 32 + q: its partner id came with level 1 (lane 48 + q)
     const int q = lane - 32;
     const bool bq = q >= 0 && q < nb && q < 16;
-    const int bjs = __shfl(lo, 48 + (lane & 15), 64);
+    // lanes 32..47 take lanes 48..63 (row-pair swap)
+    const int bjs = (int)__builtin_amdgcn_permlane16_swap((unsigned)lo, (unsigned)lo, false, false)[1];
     const int bj = (bq && bjs >= 1 && bjs <= S.N) ? bjs : i;
     double blo = 0, bhi = 0;
     {
@@ -687,11 +693,8 @@ __device__ __forceinline__ void fstage(FSys &F, const ScrTab &T, FWave &wm, FAto
                 wm.row_b[k] = b;
             }
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            mb0 |= __shfl_xor(mb0, off, 64); mb1 |= __shfl_xor(mb1, off, 64);
-            mc0 |= __shfl_xor(mc0, off, 64); mc1 |= __shfl_xor(mc1, off, 64);
-        }
+        mb0 = wave_or64(mb0); mb1 = wave_or64(mb1);
+        mc0 = wave_or64(mc0); mc1 = wave_or64(mc1);
         if (lane == 0) {
             A.rowlo = rowlo; A.rowhi = rowhi; A.spanok = spanok;
             A.mb[0] = mb0; A.mb[1] = mb1; A.mc[0] = mc0; A.mc[1] = mc1;
@@ -734,12 +737,7 @@ __device__ __forceinline__ void fstage(FSys &F, const ScrTab &T, FWave &wm, FAto
         A.s0 = s0;
         A.s1 = s1;
     }
-    int incl = cnt;
-#pragma unroll
-    for (int off = 1; off < 32; off <<= 1) {
-        const int v = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += v;
-    }
+    const int incl = scan32_incl(cnt);
     if (lane < 27) wm.coff[lane + 1] = incl;
     if (lane == 0) wm.coff[0] = 0;
     __builtin_amdgcn_wave_barrier();
@@ -1022,7 +1020,7 @@ __device__ __forceinline__ void collision_fast(FSys &F, const ScrTab &T, FWave &
         const unsigned long long m = __ballot(mink == kw && kw != 0x7fffffff);
         const int src = m ? __ffsll((long long)m) - 1 : 0;
         bcast(win, src);
-        const float w32 = __shfl(myt32, src, 64);
+        const float w32 = lane_get(myt32, src);
         if (m && lane == 0 && win.t > 0 && w32 < FINF) cnt_max(F.max_screen_err, fabs((double)w32 - win.t) / win.t);
         if (!m) win.t = __builtin_inf();
     }
@@ -1216,7 +1214,7 @@ __device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm
     if (S.thermostat) {
         double z = 0;
         if (lane == 0) z = fast_normal_at(F, i, A.rctr);
-        z = __shfl(z, 0, 64);
+        z = lane_get(z, 0);
         const double tt = S.thermoF * ((z < 0) ? (-z) : (z));
         take(best, S, tt, 0, SDMD_INVALID_INTER, SDMD_THER, 0);
     }
@@ -1276,8 +1274,8 @@ __device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm
     }
     if (hbit != SDMD_HBSKIP) {
         if (A.hbbond > 0) {
-            const double tt = __shfl(pt_t, 32, 64), pot = __shfl(pt_pot, 32, 64);
-            const int sub = __shfl(pt_sub, 32, 64);
+            const double tt = lane_get(pt_t, 32), pot = lane_get(pt_pot, 32);
+            const int sub = lane_get(pt_sub, 32);
             take(best, S, tt, pot, sub, SDMD_HB, A.hbbond);
             nfp64 += 1;
         } else if (A.role == 'A' || A.role == 'D' || A.role == 'S') {
@@ -1358,8 +1356,8 @@ __device__ __forceinline__ void predict_fast(FSys &F, const ScrTab &T, FWave &wm
     {
         int flag = 0;
         for (int q = 1; q <= nq; q++) {
-            const double tt = __shfl(pt_t, q - 1, 64), pot = __shfl(pt_pot, q - 1, 64);
-            const int sub = __shfl(pt_sub, q - 1, 64);
+            const double tt = lane_get(pt_t, q - 1), pot = lane_get(pt_pot, q - 1);
+            const int sub = lane_get(pt_sub, q - 1);
             if (flag == 0) A.nbnum = q;
             nfp64 += 1;
             if (tt > 0 && best.t > tt) {
@@ -1658,16 +1656,14 @@ __device__ __forceinline__ void block_min3(int b, bool in, double t, int p, doub
     double m = t;
     int km = k;
     wave_argmin(m, km);
-    const int pm = __shfl(p, (km - 1 - b * 64) & 63, 64);
+    const int pm = lane_get(p, (km - 1 - b * 64) & 63);
     const bool out1 = (k == km) || (k == pm && p == km);
     double u = out1 ? __builtin_inf() : t;
     int ku = out1 ? 0x7fffffff : k;
     wave_argmin(u, ku);
-    const int pu = __shfl(p, (ku - 1 - b * 64) & 63, 64);
+    const int pu = lane_get(p, (ku - 1 - b * 64) & 63);
     const bool out2 = out1 || (k == ku) || (k == pu && p == ku);
-    double w = out2 ? __builtin_inf() : t;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) w = fmin(w, __shfl_xor(w, off, 64));
+    const double w = wave_min_d(out2 ? __builtin_inf() : t);
     t1 = m; a1 = km; t2 = u; a2 = ku; t3 = w;
 }
 
@@ -1915,8 +1911,8 @@ __device__ __forceinline__ void helper_loop(FSys &F, const ScrTab &tab, FShared 
                 __builtin_amdgcn_s_sleep(2);
             }
         }
-        q = __shfl(q, 0, 64);
-        quit = __shfl(quit, 0, 64);
+        q = lane_get(q, 0);
+        quit = lane_get(quit, 0);
         if (quit) break;
         acquire_agent();                         // drop stale L1 lines: the owner's writes, from the L2
         const int a = J->a[q];
@@ -2009,11 +2005,8 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, FJobs *jobs,
             const double t = sh.b1[b];
             if (t < m1 || (t == m1 && sh.ba[b] < a1)) { m1 = t; a1 = sh.ba[b]; }
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            m3 = fmin(m3, __shfl_xor(m3, off, 64));
-            m2 = fmin(m2, __shfl_xor(m2, off, 64));
-        }
+        m3 = wave_min_d(m3);
+        m2 = wave_min_d(m2);
         wave_argmin(m1, a1);
         if (lane == 0) { sh.red_t[wave] = m2; sh.red_t3[wave] = m3; sh.red_t1[wave] = m1; sh.red_a[wave] = a1; }
         __syncthreads();
@@ -2160,8 +2153,7 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, FJobs *jobs,
                                    apart(xka, xla) && apart(xka, xlp) && apart(xkp, xla) && apart(xkp, xlp);
                 if (!indep) mine |= 1ull << k;
             }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) mine |= __shfl_xor(mine, off, 64);
+            mine = wave_or64(mine);
             if (lane == 0 && mine) atomicOr(&sh.conflict, mine);
         }
         __syncthreads();
@@ -2258,7 +2250,7 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, FJobs *jobs,
                     r = job_claim(J);
                 }
             }
-            r = __shfl(r, 0, 64);
+            r = lane_get(r, 0);
             if (r < 0) break;
             const int q = ren_at(r);
             predict_fast<FQ>(F, tab, sh.w[wave], sh.ren_a[q], sh.ren_t[q], ws, &err);
@@ -2291,17 +2283,13 @@ __global__ __launch_bounds__(64 * W) void k_fast_run(FSys *systems, FJobs *jobs,
         // the next round)
         if (wave == 0) {
             const unsigned int m = (lane < FMAXBLK / 32) ? sh.dmask[lane] : 0u;
+            static_assert(FMAXBLK / 32 <= 32, "the touched-block mask fits lanes 0..31");
             const int c = __popc(m);
-            int incl = c;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const int v = __shfl_up(incl, off, 64);
-                if (lane >= off) incl += v;
-            }
+            const int incl = scan32_incl(c);
             int o = incl - c;
             for (unsigned int mm = m; mm; mm &= mm - 1) sh.dirty[o++] = lane * 32 + __builtin_ctz(mm);
             if (lane < FMAXBLK / 32) sh.dmask[lane] = 0u;
-            if (lane == 63) sh.ndirty = incl;
+            if (lane == 31) sh.ndirty = incl;
         }
         __syncthreads();
         PROF_MARK(S, FPH_EXEC, pt);
