@@ -264,6 +264,7 @@ struct FWave {
     int dyn_j[SDMD_DYNSLOTS], dyn_b[SDMD_DYNSLOTS];
     int coff[28];
     int cell[27];
+    fvec4 cof[27];               // neighbour cell c's offset times the cell size (x, y, z)
     // the predicted atom's constraint shells (FP32), CONSTRn lists
     int con_j[FCON], con_nstep[FCON];
     unsigned int con_ez[FCON];
@@ -464,14 +465,17 @@ __device__ __forceinline__ Scr screen_shell(const Shell &c, const R &rad, bool c
 
 // compact FP32 copy of the collision table (RightPair resolved), held in LDS
 constexpr int FRAD = 8192;
+// indexed by (type_i, type_j) directly: the entry of the resolved pair,
+// with that pair's index (one LDS read per candidate)
 struct ScrPair {
     float dmin, dmax;
     unsigned short off;          // first radius in the FP32 pool
     unsigned short nstep;
-    unsigned int ezero;
+    unsigned short ezero;        // bit k: step k carries no energy (nstep <= 16)
+    short rp;                    // resolved pair (RightPair), -1 = none
 };
+static_assert(sizeof(ScrPair) == 16, "one 16-byte LDS read per screen table entry");
 struct ScrTab {
-    short rp[NT * NT];           // resolved pair per (type_i, type_j), -1 = none
     ScrPair pr[NT * NT];
     float rad[FRAD];
     // per-lane gather map of a predicted atom's fields: lane L loads the
@@ -725,7 +729,15 @@ __device__ __forceinline__ void fstage(FSys &F, const ScrTab &T, FWave &wm, FAto
         const int staged = (ncon <= FCON) && !__ballot(!ok);
         if (lane == 0) A.constaged = staged;
     }
-    if (lane < 27) wm.cell[lane] = cell;
+    if (lane < 27) {
+        wm.cell[lane] = cell;
+        fvec4 co;
+        co.x = c_OX[lane] * F.L32[0];
+        co.y = c_OY[lane] * F.L32[1];
+        co.z = c_OZ[lane] * F.L32[2];
+        co.w = 0.f;
+        wm.cof[lane] = co;
+    }
     // own mirror record, as put_record writes it for a state at time t
     if (lane == 0) {
         fvec4 s0, s1;
@@ -804,9 +816,10 @@ __device__ __forceinline__ void screen_at(const FSys &F, const ScrTab &T, const 
     o.bits = bits;
     if ((bits & SDMD_BOND_CONNECT) || A.hbbond == j || fin_nb_list(A, j)) return;
     const float dtj = tA - r0.w;
-    const float rx = s0.x - (r0.x + r1.x * dtj) - c_OX[c] * F.L32[0];
-    const float ry = s0.y - (r0.y + r1.y * dtj) - c_OY[c] * F.L32[1];
-    const float rz = s0.z - (r0.z + r1.z * dtj) - c_OZ[c] * F.L32[2];
+    const fvec4 co = wm.cof[c];
+    const float rx = s0.x - (r0.x + r1.x * dtj) - co.x;
+    const float ry = s0.y - (r0.y + r1.y * dtj) - co.y;
+    const float rz = s0.z - (r0.z + r1.z * dtj) - co.z;
     const float rr = rx * rx + ry * ry + rz * rz;
     if (rr > calib + cband) return;
     o.live = true;
@@ -846,12 +859,12 @@ __device__ __forceinline__ void screen_at(const FSys &F, const ScrTab &T, const 
             sc = screen_shell(c, [&](int k) { return (float)G(S.sd)[s0 + k]; }, true, rr, b, v2);
         }
     } else {
-        const int ri = T.rp[A.type * NT + tj];
+        const ScrPair sp = T.pr[A.type * NT + tj];
+        const int ri = sp.rp;
         o.rp = ri;
         if (ri < 0) {
             sc.flag = true; sc.t = FINF; sc.e = 0;
         } else {
-            const ScrPair &sp = T.pr[ri];
             Shell c;
             c.dmin = sp.dmin; c.dmax = sp.dmax; c.nstep = sp.nstep; c.ezero = sp.ezero;
             const float *rad = T.rad + sp.off;
@@ -932,9 +945,10 @@ __device__ __forceinline__ void collision_fast(FSys &F, const ScrTab &T, FWave &
                 bool nr = false;
                 if (k < n && j >= 1 && j <= S.N && j != A.i) {
                     const float dtj = tA - r0[q].w;
-                    const float rx = s0.x - (r0[q].x + r1[q].x * dtj) - c_OX[cq[q]] * F.L32[0];
-                    const float ry = s0.y - (r0[q].y + r1[q].y * dtj) - c_OY[cq[q]] * F.L32[1];
-                    const float rz = s0.z - (r0[q].z + r1[q].z * dtj) - c_OZ[cq[q]] * F.L32[2];
+                    const fvec4 co = wm.cof[cq[q]];
+                    const float rx = s0.x - (r0[q].x + r1[q].x * dtj) - co.x;
+                    const float ry = s0.y - (r0[q].y + r1[q].y * dtj) - co.y;
+                    const float rz = s0.z - (r0[q].z + r1[q].z * dtj) - co.z;
                     nr = rx * rx + ry * ry + rz * rz <= calib + cband;
                 }
                 const unsigned long long m = __ballot(nr);
@@ -2482,31 +2496,38 @@ int sdmd_fast_open(sdmd_gpu_ctx *c, const sdmd_gpu_opts *o) {
         auto back = [](int t) { return t == 3 ? 1 : t == 4 ? 2 : t == 27 ? 26 : t; };
         int pool = 0;
         rc = SDMD_OK;
-        for (int t1 = 0; t1 < NT; t1++)
-            for (int t2 = 0; t2 < NT; t2++) {
-                const int a = back(t1), b = back(t2);
-                const int cand[4] = {t1 * NT + t2, a * NT + t2, t1 * NT + b, a * NT + b};
-                short r = -1;
-                for (int q = 0; q < 4 && r < 0; q++)
-                    if (d->pair[cand[q]].dmin > 0) r = (short)cand[q];
-                tb->rp[t1 * NT + t2] = r;
-            }
+        // every defined pair's entry, radii in the pool once
+        ScrPair *ent = (ScrPair *)calloc(NT * NT, sizeof(ScrPair));
+        if (!ent) { free(tb); return SDMD_ENOMEM; }
         for (int p = 0; p < NT * NT; p++) {
             const sdmd_pair &pp = d->pair[p];
+            ent[p].rp = -1;
             if (!(pp.dmin > 0)) continue;
             if (pp.nstep > 16 || pool + pp.nstep > FRAD) { rc = SDMD_EUNSUPPORTED; break; }
-            ScrPair &sp = tb->pr[p];
+            ScrPair &sp = ent[p];
             sp.dmin = (float)pp.dmin;
             sp.dmax = (float)pp.dmax;
             sp.nstep = (unsigned short)pp.nstep;
             sp.off = (unsigned short)pool;
             sp.ezero = 0;
+            sp.rp = (short)p;
             for (int k = 0; k < pp.nstep; k++) {
                 tb->rad[pool + k] = (float)d->step_d[pp.step0 + k];
-                if (d->step_e[pp.step0 + k] == 0) sp.ezero |= 1u << k;
+                if (d->step_e[pp.step0 + k] == 0) sp.ezero |= (unsigned short)(1u << k);
             }
             pool += pp.nstep;
         }
+        for (int t1 = 0; t1 < NT; t1++)
+            for (int t2 = 0; t2 < NT; t2++) {
+                const int a = back(t1), b = back(t2);
+                const int cand[4] = {t1 * NT + t2, a * NT + t2, t1 * NT + b, a * NT + b};
+                int r = -1;
+                for (int q = 0; q < 4 && r < 0; q++)
+                    if (d->pair[cand[q]].dmin > 0) r = cand[q];
+                if (r >= 0) tb->pr[t1 * NT + t2] = ent[r];
+                else tb->pr[t1 * NT + t2].rp = -1;
+            }
+        free(ent);
         // the gather map of fstage's first level (device addresses of this
         // system's arrays)
         {
